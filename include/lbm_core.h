@@ -1,0 +1,171 @@
+/*
+ * lbm_core.h -- C ABI of the MI355X lattice-Boltzmann collide-stream core.
+ *
+ * One context owns one D3Q27 lattice in HBM (single in-place A-A population
+ * array, structure-of-arrays [q][z][y][x]) plus one HIP stream.  The caller
+ * owns every host pointer; the library copies what it needs.  No exception
+ * crosses this boundary; every entry point returns an lbm_status.  A context
+ * is not thread-safe: drive it from one host thread.
+ *
+ * The per-step operator sequence replaced by lbm_step() is the reference's
+ *   Turbulence::omega -> Collision -> StreamingAB -> BounceBackWallHalf
+ *   -> Inlet -> Outlet -> Periodic -> MomentDensityVelocityN27
+ * as called from SolverTurbulentLES::runSimulation
+ * (3D/simulation/src/solvers/SolverTurbulentLES.h:184-214) and
+ * SolverLaminar::runSimulation (3D/simulation/src/solvers/SolverLaminar.h:132-157).
+ * Each reference policy is a `static void op(LBMDataPointer&, LBMConstantsPointer&)`
+ * (e.g. CollisionCumD3Q27Turbulent2015.h:19, StreamingAB.h:19,
+ * BounceBackWallHalf.h:19, OmegaLES.h:25, MomentDensityVelocityN27.h:19); here
+ * the whole sequence is one fused kernel per time step.
+ */
+#ifndef LBM_CORE_H
+#define LBM_CORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_Q 27
+
+typedef struct lbm_ctx lbm_ctx;
+
+typedef enum lbm_status {
+    LBM_OK = 0,
+    LBM_E_ARG = 1,          /* invalid argument / inconsistent flag grid */
+    LBM_E_HIP = 2,          /* HIP runtime error (no device, launch failure) */
+    LBM_E_OOM = 3,          /* device allocation failed */
+    LBM_E_NAN = 4,          /* non-finite mass or momentum detected */
+    LBM_E_STATE = 5,        /* call out of order (e.g. step before init) */
+    LBM_E_UNSUPPORTED = 6   /* valid request that this build does not implement */
+} lbm_status;
+
+/* Collision operator (reference: collisions/CollisionSRT*.h, CollisionCumD3Q27*2015.h). */
+enum { LBM_COLLISION_SRT = 0, LBM_COLLISION_CUM2015 = 1 };
+
+/* Relaxation-rate closure (reference: turbulence/OmegaNo.h, turbulence/OmegaLES.h).
+ * LBM_CLOSURE_MLP is the learned sub-grid closure (no reference counterpart). */
+enum { LBM_CLOSURE_NONE = 0, LBM_CLOSURE_SMAGORINSKY = 1, LBM_CLOSURE_MLP = 2 };
+
+/* Population storage in HBM.  F16DEV/BF16DEV store g_i = f_i - w_i*rho0 and do
+ * all collision arithmetic in FP32. */
+enum { LBM_STORAGE_F64 = 0, LBM_STORAGE_F32 = 1, LBM_STORAGE_F16DEV = 2, LBM_STORAGE_BF16DEV = 3 };
+
+/* Collision arithmetic.  LITERAL follows the reference's operation order
+ * (three-pass central-moment transform, full cumulant set); FAST uses the
+ * algebraically equivalent reduced form (omega_2..omega_10 = 1 makes every
+ * post-collision cumulant of order >= 3 vanish). */
+enum { LBM_ARITH_LITERAL = 0, LBM_ARITH_FAST = 1 };
+
+/* Compatibility switches reproducing reference quirks. */
+#define LBM_COMPAT_CREV20    0x1u  /* D3Q27.h:120-124 c_rev[20] = 21 (correct: 19)          */
+#define LBM_COMPAT_F32CONST  0x2u  /* D3Q27.h:113-118 float weights; Cum2015.h:280-282 1.f/3.f */
+#define LBM_COMPAT_REFERENCE (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST)
+
+/* Boundary-node kinds.  The flag grid keeps the reference encoding
+ * (geometryMesherBoundary.h): 0 solid, >0 fluid (1 + number of solid
+ * neighbours), -1 inlet, -2 outlet, -3 periodic ghost. */
+enum {
+    LBM_BC_WALL = 0,                 /* BounceBackWallHalf.h:29-52, wall list        */
+    LBM_BC_INLET_MOVING_WALL = 1,    /* InletVelocityMovingWall.h:31-53              */
+    LBM_BC_INLET_EQUILIBRIUM = 2,    /* InletVelocityEquilibrium.h:51-103            */
+    LBM_BC_OUTLET_INTERPOLATED = 3,  /* OutletDensityInterpolated.h:170-252          */
+    LBM_BC_PERIODIC_GHOST = 4        /* Periodic.h:43-138                            */
+};
+
+/* One boundary-list entry; mirrors the reference PODs boundaryConditionWall /
+ * Inlet / Outlet / Periodic (traits/LBMTraits.h:17-61).  Normals, velocities
+ * and densities are float there, so they are float here. */
+typedef struct lbm_bc_node {
+    int32_t x, y, z;
+    int32_t kind;          /* LBM_BC_* */
+    float nx, ny, nz;      /* outward normal */
+    float ux, uy, uz;      /* inlet velocity, lattice units */
+    float rho;             /* outlet target density, lattice units */
+    int32_t per_idx;       /* periodic partner index along the normal axis */
+    int32_t regular;       /* reference 'regular' bit */
+} lbm_bc_node;
+
+typedef struct lbm_desc {
+    int32_t nx, ny, nz;        /* lattice size including any solid halo */
+    int32_t periodic[3];       /* 1 = index-wrap periodicity along x/y/z */
+    int32_t collision;         /* LBM_COLLISION_* */
+    int32_t closure;           /* LBM_CLOSURE_* */
+    int32_t storage;           /* LBM_STORAGE_* */
+    int32_t arith;             /* LBM_ARITH_* */
+    uint32_t compat;           /* LBM_COMPAT_* bits */
+    int32_t n_bc;              /* entries in bc[] */
+    double tau;                /* molecular relaxation time, omega = 1/tau (LBMConstants::tau) */
+    double nu;                 /* lattice viscosity used by the equilibrium inlet (LBMConstants::ny) */
+    double c_les;              /* Smagorinsky constant in the reference formula (LBMConstants::CLES) */
+    double rho0;               /* reference density of the deviation storage */
+    const int32_t* flags;      /* host, reference encoding, x fastest, nx*ny*nz; copied */
+    const lbm_bc_node* bc;     /* host, copied; may be NULL when n_bc == 0 */
+} lbm_desc;
+
+/* Create a context on the current HIP device.  Validates the flag grid: every
+ * non-solid cell must have all 26 neighbours inside the array or across an
+ * index-wrap axis (the reference has no bounds checks, StreamingAB.h:51-56). */
+lbm_status lbm_create(const lbm_desc* desc, lbm_ctx** out);
+
+/* Equilibrium initialisation f = f_eq(rho, u) on every non-solid cell
+ * (InitializationEquilibriumConstVector.h:74-126).  NULL rho means rho0
+ * everywhere; NULL velocity components mean 0.  Arrays are nx*ny*nz, x fastest.
+ * The next lbm_step uses these rho/u for its first collision, as the reference
+ * does (its moments are only recomputed at the end of a step). */
+lbm_status lbm_init_equilibrium(lbm_ctx* ctx, const double* rho, const double* ux,
+                                const double* uy, const double* uz);
+
+/* Overwrite the population state (layout [q][z][y][x], FP64).  The next step
+ * derives rho/u from these populations. */
+lbm_status lbm_set_populations(lbm_ctx* ctx, const double* f);
+
+/* Advance nsteps time steps; asynchronous on the context stream. */
+lbm_status lbm_step(lbm_ctx* ctx, int32_t nsteps);
+
+/* Wait for queued work; returns LBM_E_NAN if a diagnostic found NaN/Inf. */
+lbm_status lbm_sync(lbm_ctx* ctx);
+
+/* rho and u of every non-solid cell exactly as MomentDensityVelocityN27
+ * (moments/MomentDensityVelocityN27.h:65-81) computes them from the current
+ * populations; 0 on solid cells.  Any pointer may be NULL.  Synchronises. */
+lbm_status lbm_get_macroscopic(lbm_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
+
+/* Current (post-stream, post-boundary) populations, [q][z][y][x], FP64;
+ * 0 on solid cells.  Synchronises. */
+lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
+
+/* Relaxation rate omega that the next step will use on every cell, computed
+ * from the current state as Turbulence::omega does at the start of a step
+ * (OmegaLES.h:69-131 / OmegaNo.h:34-47); 0 on solid cells.  Synchronises. */
+lbm_status lbm_get_omega(lbm_ctx* ctx, double* omega);
+
+/* FP64 reductions over non-solid cells of the current state:
+ * out = {sum rho, sum rho*ux, sum rho*uy, sum rho*uz}.  Synchronises;
+ * returns LBM_E_NAN if any is non-finite. */
+lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
+
+/* Steps completed so far. */
+lbm_status lbm_step_count(lbm_ctx* ctx, int64_t* steps);
+
+/* Average device time of the collide-stream kernel over the last lbm_step
+ * call, in milliseconds (HIP events on the context stream). */
+lbm_status lbm_last_kernel_ms(lbm_ctx* ctx, double* ms);
+
+/* Bytes of device memory held by the context. */
+lbm_status lbm_device_bytes(lbm_ctx* ctx, size_t* bytes);
+
+/* Number of visible HIP devices, and the name / gfx target of the current
+ * one (name may be NULL).  Returns LBM_E_HIP when no device is visible. */
+lbm_status lbm_device_info(int32_t* count, char* name, size_t name_len);
+
+const char* lbm_status_str(lbm_status s);
+void lbm_destroy(lbm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_CORE_H */

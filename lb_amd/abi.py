@@ -1,0 +1,106 @@
+"""ctypes view of the C ABI declared in include/lbm_core.h.
+
+The HIP core is a shared library built in-tree (lb_amd/liblbm_core_hip.so).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORE_LIB = os.path.join(HERE, "liblbm_core_hip.so")
+HOST_LIB = os.path.join(HERE, "liblbm_host.so")
+
+Q = 27
+
+# status codes
+LBM_OK, LBM_E_ARG, LBM_E_HIP, LBM_E_OOM, LBM_E_NAN, LBM_E_STATE, LBM_E_UNSUPPORTED = range(7)
+STATUS_NAMES = {0: "LBM_OK", 1: "LBM_E_ARG", 2: "LBM_E_HIP", 3: "LBM_E_OOM", 4: "LBM_E_NAN",
+                5: "LBM_E_STATE", 6: "LBM_E_UNSUPPORTED"}
+
+COLLISION_SRT, COLLISION_CUM2015 = 0, 1
+CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_MLP = 0, 1, 2
+STORAGE_F64, STORAGE_F32, STORAGE_F16DEV, STORAGE_BF16DEV = 0, 1, 2, 3
+ARITH_LITERAL, ARITH_FAST = 0, 1
+COMPAT_CREV20, COMPAT_F32CONST = 0x1, 0x2
+COMPAT_REFERENCE = COMPAT_CREV20 | COMPAT_F32CONST
+BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
+
+STORAGE_BYTES = {STORAGE_F64: 8, STORAGE_F32: 4, STORAGE_F16DEV: 2, STORAGE_BF16DEV: 2}
+STORAGE_NAMES = {STORAGE_F64: "f64", STORAGE_F32: "f32", STORAGE_F16DEV: "f16dev", STORAGE_BF16DEV: "bf16dev"}
+
+
+class BcNode(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("z", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("nx", ctypes.c_float), ("ny", ctypes.c_float), ("nz", ctypes.c_float),
+                ("ux", ctypes.c_float), ("uy", ctypes.c_float), ("uz", ctypes.c_float),
+                ("rho", ctypes.c_float), ("per_idx", ctypes.c_int32), ("regular", ctypes.c_int32)]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3),
+                ("collision", ctypes.c_int32), ("closure", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("arith", ctypes.c_int32), ("compat", ctypes.c_uint32), ("n_bc", ctypes.c_int32),
+                ("tau", ctypes.c_double), ("nu", ctypes.c_double), ("c_les", ctypes.c_double),
+                ("rho0", ctypes.c_double),
+                ("flags", ctypes.POINTER(ctypes.c_int32)), ("bc", ctypes.POINTER(BcNode))]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes) of every entry point declared in include/lbm_core.h
+CORE_SYMBOLS = [
+    ("lbm_create", ctypes.c_int, [ctypes.POINTER(Desc), ctypes.POINTER(_vp)]),
+    ("lbm_init_equilibrium", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("lbm_set_populations", ctypes.c_int, [_vp, _dp]),
+    ("lbm_step", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("lbm_sync", ctypes.c_int, [_vp]),
+    ("lbm_get_macroscopic", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    ("lbm_get_populations", ctypes.c_int, [_vp, _dp]),
+    ("lbm_get_omega", ctypes.c_int, [_vp, _dp]),
+    ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
+    ("lbm_step_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("lbm_last_kernel_ms", ctypes.c_int, [_vp, _dp]),
+    ("lbm_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("lbm_device_info", ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]),
+    ("lbm_status_str", ctypes.c_char_p, [ctypes.c_int]),
+    ("lbm_destroy", None, [_vp]),
+]
+
+_core = None
+
+
+def load_core() -> ctypes.CDLL:
+    """Load the HIP core.  Raises if it has not been built."""
+    global _core
+    if _core is None:
+        if not os.path.exists(CORE_LIB):
+            raise RuntimeError(f"HIP core {CORE_LIB} is not built; run __graft_entry__.build()")
+        lib = ctypes.CDLL(CORE_LIB)
+        for name, res, args in CORE_SYMBOLS:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _core = lib
+    return _core
+
+
+def device_info():
+    """(count, description) of the visible HIP devices; (0, reason) when none."""
+    lib = load_core()
+    n = ctypes.c_int32(0)
+    buf = ctypes.create_string_buffer(256)
+    st = lib.lbm_device_info(ctypes.byref(n), buf, 256)
+    if st != LBM_OK:
+        return 0, STATUS_NAMES.get(st, str(st))
+    return n.value, buf.value.decode(errors="replace")
+
+
+def ptr(a, ctype=ctypes.c_double):
+    """ctypes pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

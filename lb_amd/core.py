@@ -1,0 +1,127 @@
+"""Python handle on one lattice context of the HIP core (C ABI: include/lbm_core.h)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        lib = abi.load_core()
+        msg = lib.lbm_status_str(status).decode()
+        super().__init__(f"{where}: {abi.STATUS_NAMES.get(status, status)} ({msg})")
+        self.status = status
+
+
+def _check(status: int, where: str):
+    if status != abi.LBM_OK:
+        raise LbmError(status, where)
+
+
+def make_desc(flags: np.ndarray, bc=None, *, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_NONE,
+              storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE,
+              periodic=(0, 0, 0), tau=0.6, nu=0.0, c_les=0.0, rho0=1.0):
+    """Build an lbm_desc.  flags: int32 array shaped (nz, ny, nx) (x fastest in memory)."""
+    flags = np.ascontiguousarray(flags, dtype=np.int32)
+    nz, ny, nx = flags.shape
+    d = abi.Desc()
+    d.nx, d.ny, d.nz = nx, ny, nz
+    d.periodic[:] = [int(bool(p)) for p in periodic]
+    d.collision, d.closure, d.storage, d.arith, d.compat = collision, closure, storage, arith, compat
+    d.tau, d.nu, d.c_les, d.rho0 = float(tau), float(nu), float(c_les), float(rho0)
+    d.flags = flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    bc = bc if bc is not None else []
+    if len(bc):
+        arr = (abi.BcNode * len(bc))(*bc)
+        d.bc = ctypes.cast(arr, ctypes.POINTER(abi.BcNode))
+    else:
+        arr = None
+        d.bc = None
+    d.n_bc = len(bc)
+    # keep the backing buffers alive with the descriptor
+    d._keep = (flags, arr)
+    return d
+
+
+class Lattice:
+    """One D3Q27 lattice resident in HBM, advanced by the fused collide-stream kernel."""
+
+    def __init__(self, desc: abi.Desc):
+        self._lib = abi.load_core()
+        self.desc = desc
+        self.shape = (desc.nz, desc.ny, desc.nx)
+        self.n = desc.nx * desc.ny * desc.nz
+        h = ctypes.c_void_p()
+        _check(self._lib.lbm_create(ctypes.byref(desc), ctypes.byref(h)), "lbm_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state ------------------------------------------------------------
+    def init_equilibrium(self, rho=None, ux=None, uy=None, uz=None):
+        arrs = [None if a is None else np.ascontiguousarray(np.broadcast_to(a, self.shape), dtype=np.float64)
+                for a in (rho, ux, uy, uz)]
+        _check(self._lib.lbm_init_equilibrium(self._h, *[abi.ptr(a) for a in arrs]), "lbm_init_equilibrium")
+
+    def set_populations(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(abi.Q, *self.shape)
+        _check(self._lib.lbm_set_populations(self._h, abi.ptr(f)), "lbm_set_populations")
+
+    def step(self, n: int = 1):
+        _check(self._lib.lbm_step(self._h, int(n)), "lbm_step")
+
+    def sync(self):
+        _check(self._lib.lbm_sync(self._h), "lbm_sync")
+
+    def macroscopic(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        _check(self._lib.lbm_get_macroscopic(self._h, *[abi.ptr(a) for a in out]), "lbm_get_macroscopic")
+        return out
+
+    def populations(self):
+        f = np.empty((abi.Q, *self.shape))
+        _check(self._lib.lbm_get_populations(self._h, abi.ptr(f)), "lbm_get_populations")
+        return f
+
+    def omega(self):
+        w = np.empty(self.shape)
+        _check(self._lib.lbm_get_omega(self._h, abi.ptr(w)), "lbm_get_omega")
+        return w
+
+    def conservation(self):
+        out = np.empty(4)
+        _check(self._lib.lbm_conservation(self._h, abi.ptr(out)), "lbm_conservation")
+        return out
+
+    def steps(self) -> int:
+        v = ctypes.c_int64()
+        _check(self._lib.lbm_step_count(self._h, ctypes.byref(v)), "lbm_step_count")
+        return v.value
+
+    def last_kernel_ms(self) -> float:
+        v = ctypes.c_double()
+        _check(self._lib.lbm_last_kernel_ms(self._h, ctypes.byref(v)), "lbm_last_kernel_ms")
+        return v.value
+
+    def device_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        _check(self._lib.lbm_device_bytes(self._h, ctypes.byref(v)), "lbm_device_bytes")
+        return v.value

@@ -1,0 +1,335 @@
+/*
+ * lbm_oracle.c -- CPU oracle of the reference collide-stream path.
+ * TEST INFRASTRUCTURE ONLY (see lbm_oracle.h): the checker for the HIP core
+ * and the CPU baseline of bench.py; never linked into the product.
+ *
+ * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* D3Q27 velocity set in the reference's slot order
+ * (3D/simulation/src/solvers/models/D3Q27/D3Q27.h:81-111).  The order defines
+ * the population layout, so it is kept. */
+static const int ORC_C[27][3] = {
+    {0, 0, 0},
+    {1, 0, 0}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 1}, {0, -1, 0}, {0, 1, 0},
+    {1, 0, -1}, {-1, 0, 1}, {1, 0, 1}, {-1, 0, -1}, {-1, -1, 0}, {1, 1, 0},
+    {0, 1, -1}, {0, -1, 1}, {-1, 1, 0}, {1, -1, 0}, {0, 1, 1}, {0, -1, -1},
+    {-1, 1, -1}, {1, -1, 1}, {-1, -1, 1}, {1, 1, -1}, {1, -1, -1}, {-1, 1, 1}, {-1, -1, -1}, {1, 1, 1}};
+
+/* Opposite direction, computed from ORC_C (the corrected table). */
+static int orc_opposite(int q)
+{
+    for (int p = 0; p < 27; ++p)
+        if (ORC_C[p][0] == -ORC_C[q][0] && ORC_C[p][1] == -ORC_C[q][1] && ORC_C[p][2] == -ORC_C[q][2]) return p;
+    return -1;
+}
+
+/* Reverse table: the reference's c_rev (D3Q27.h:120-124) differs from the
+ * true opposite only at slot 20, where it holds 21 (quirk Q1). */
+static void orc_crev(uint32_t compat, int* crev)
+{
+    for (int q = 0; q < 27; ++q) crev[q] = orc_opposite(q);
+    if (compat & LBM_COMPAT_CREV20) crev[20] = 21;
+}
+
+/* Weight class of a direction: number of non-zero components. */
+static int orc_nnz(int q) { return (ORC_C[q][0] != 0) + (ORC_C[q][1] != 0) + (ORC_C[q][2] != 0); }
+
+/* ---------------- double instantiation ---------------- */
+#define REAL double
+#define FN(x) x##_d
+#define ORC_SQRT sqrt
+#include "lbm_oracle_impl.h"
+#undef REAL
+#undef FN
+#undef ORC_SQRT
+
+/* ---------------- float instantiation ---------------- */
+#define REAL float
+#define FN(x) x##_f
+#define ORC_SQRT sqrtf
+#include "lbm_oracle_impl.h"
+#undef REAL
+#undef FN
+#undef ORC_SQRT
+
+struct orc {
+    int dbl;
+    double rho0;
+    state_d d;
+    state_f f;
+};
+
+static int g_threads = 0;
+
+void orc_set_threads(int n)
+{
+    g_threads = n;
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Weights: D3Q27.h:113-118 writes them as float quotients (8.f/27.f ...),
+ * which survive RealType = double (quirk Q2). */
+#define ORC_FILL_CONSTS(S, T)                                                              \
+    do {                                                                                   \
+        for (int q = 0; q < 27; ++q) {                                                     \
+            int k = orc_nnz(q);                                                            \
+            if (compat & LBM_COMPAT_F32CONST)                                              \
+                (S)->w[q] = (T)(k == 0 ? 8.f / 27.f : k == 1 ? 2.f / 27.f : k == 2 ? 1.f / 54.f : 1.f / 216.f); \
+            else                                                                           \
+                (S)->w[q] = (T)(k == 0 ? 8.0 / 27.0 : k == 1 ? 2.0 / 27.0 : k == 2 ? 1.0 / 54.0 : 1.0 / 216.0); \
+        }                                                                                  \
+        orc_crev(compat, (S)->crev);                                                       \
+        (S)->c13 = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);       \
+        (S)->cs = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / sqrtf(3.f)) : (T)(1.0 / sqrt(3.0)); \
+    } while (0)
+
+#define ORC_ALLOC_STATE(S, T)                                                               \
+    do {                                                                                    \
+        (S)->nx = d->nx; (S)->ny = d->ny; (S)->nz = d->nz; (S)->N = N;                      \
+        (S)->per[0] = d->periodic[0] != 0; (S)->per[1] = d->periodic[1] != 0;               \
+        (S)->per[2] = d->periodic[2] != 0;                                                  \
+        (S)->collision = d->collision; (S)->closure = d->closure; (S)->compat = compat;     \
+        (S)->tau = (T)d->tau; (S)->nu = (T)d->nu; (S)->CLES = (T)d->c_les;                  \
+        ORC_FILL_CONSTS(S, T);                                                              \
+        (S)->flags = (int*)malloc(sizeof(int) * N);                                         \
+        (S)->df = (T*)calloc((size_t)27 * N, sizeof(T));                                    \
+        (S)->df_post = (T*)calloc((size_t)27 * N, sizeof(T));                               \
+        (S)->rho = (T*)calloc((size_t)N, sizeof(T));                                        \
+        (S)->u = (T*)calloc((size_t)3 * N, sizeof(T));                                      \
+        (S)->u_err = (T*)calloc((size_t)3 * N, sizeof(T));                                  \
+        (S)->omega = (T*)calloc((size_t)N, sizeof(T));                                      \
+        (S)->n_bc = d->n_bc;                                                                \
+        (S)->bc = (lbm_bc_node*)malloc(sizeof(lbm_bc_node) * (d->n_bc > 0 ? d->n_bc : 1));  \
+        if (!(S)->flags || !(S)->df || !(S)->df_post || !(S)->rho || !(S)->u || !(S)->u_err || \
+            !(S)->omega || !(S)->bc) {                                                      \
+            orc_destroy(o);                                                                 \
+            return LBM_E_OOM;                                                               \
+        }                                                                                   \
+        memcpy((S)->flags, d->flags, sizeof(int) * N);                                      \
+        if (d->n_bc > 0) memcpy((S)->bc, d->bc, sizeof(lbm_bc_node) * d->n_bc);             \
+    } while (0)
+
+int orc_create(const lbm_desc* d, orc** out)
+{
+    if (!d || !out || !d->flags || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
+    if (d->n_bc > 0 && !d->bc) return LBM_E_ARG;
+    if (d->collision != LBM_COLLISION_SRT && d->collision != LBM_COLLISION_CUM2015) return LBM_E_ARG;
+    if (d->closure != LBM_CLOSURE_NONE && d->closure != LBM_CLOSURE_SMAGORINSKY) return LBM_E_UNSUPPORTED;
+    orc* o = (orc*)calloc(1, sizeof(orc));
+    if (!o) return LBM_E_OOM;
+    long N = (long)d->nx * d->ny * d->nz;
+    uint32_t compat = d->compat;
+    o->dbl = d->storage == LBM_STORAGE_F64;
+    o->rho0 = d->rho0;
+    if (o->dbl)
+        ORC_ALLOC_STATE(&o->d, double);
+    else
+        ORC_ALLOC_STATE(&o->f, float);
+    *out = o;
+    return LBM_OK;
+}
+
+#define ORC_FREE_STATE(S)                                                                   \
+    do {                                                                                    \
+        free((S)->flags); free((S)->df); free((S)->df_post); free((S)->rho); free((S)->u);   \
+        free((S)->u_err); free((S)->omega); free((S)->bc);                                  \
+    } while (0)
+
+void orc_destroy(orc* o)
+{
+    if (!o) return;
+    ORC_FREE_STATE(&o->d);
+    ORC_FREE_STATE(&o->f);
+    free(o);
+}
+
+int orc_init_equilibrium(orc* o, const double* rho, const double* ux, const double* uy, const double* uz)
+{
+    if (!o) return LBM_E_ARG;
+    if (o->dbl)
+        init_eq_d(&o->d, rho, ux, uy, uz, o->rho0);
+    else
+        init_eq_f(&o->f, rho, ux, uy, uz, o->rho0);
+    return LBM_OK;
+}
+
+int orc_set_populations(orc* o, const double* f)
+{
+    if (!o || !f) return LBM_E_ARG;
+    if (o->dbl)
+        set_pop_d(&o->d, f);
+    else
+        set_pop_f(&o->f, f);
+    return LBM_OK;
+}
+
+int orc_step(orc* o, int nsteps)
+{
+    if (!o || nsteps < 0) return LBM_E_ARG;
+    for (int k = 0; k < nsteps; ++k) {
+        if (o->dbl)
+            step_d(&o->d);
+        else
+            step_f(&o->f);
+    }
+    return LBM_OK;
+}
+
+#define ORC_COPY_OUT(S)                                                                     \
+    do {                                                                                    \
+        for (long i = 0; i < (S)->N; ++i) {                                                 \
+            if (rho) rho[i] = (S)->rho[i];                                                  \
+            if (ux) ux[i] = (S)->u[3 * i];                                                  \
+            if (uy) uy[i] = (S)->u[3 * i + 1];                                              \
+            if (uz) uz[i] = (S)->u[3 * i + 2];                                              \
+        }                                                                                   \
+    } while (0)
+
+int orc_get_macroscopic(orc* o, double* rho, double* ux, double* uy, double* uz)
+{
+    if (!o) return LBM_E_ARG;
+    if (o->dbl)
+        ORC_COPY_OUT(&o->d);
+    else
+        ORC_COPY_OUT(&o->f);
+    return LBM_OK;
+}
+
+int orc_get_populations(orc* o, double* f)
+{
+    if (!o || !f) return LBM_E_ARG;
+    if (o->dbl) {
+        for (long i = 0; i < 27 * o->d.N; ++i) f[i] = o->d.df[i];
+    } else {
+        for (long i = 0; i < 27 * o->f.N; ++i) f[i] = o->f.df[i];
+    }
+    return LBM_OK;
+}
+
+int orc_get_post_populations(orc* o, double* f)
+{
+    if (!o || !f) return LBM_E_ARG;
+    if (o->dbl) {
+        for (long i = 0; i < 27 * o->d.N; ++i) f[i] = o->d.df_post[i];
+    } else {
+        for (long i = 0; i < 27 * o->f.N; ++i) f[i] = o->f.df_post[i];
+    }
+    return LBM_OK;
+}
+
+int orc_get_omega(orc* o, double* w)
+{
+    if (!o || !w) return LBM_E_ARG;
+    /* the rate the next step would use: Turbulence::omega on the current state */
+    if (o->dbl) {
+        op_omega_d(&o->d);
+        for (long i = 0; i < o->d.N; ++i) w[i] = o->d.flags[i] != 0 ? o->d.omega[i] : 0.0;
+    } else {
+        op_omega_f(&o->f);
+        for (long i = 0; i < o->f.N; ++i) w[i] = o->f.flags[i] != 0 ? o->f.omega[i] : 0.0;
+    }
+    return LBM_OK;
+}
+
+int orc_error_quadratic(orc* o, double* err)
+{
+    if (!o || !err) return LBM_E_ARG;
+    *err = o->dbl ? error_quadratic_d(&o->d) : error_quadratic_f(&o->f);
+    return LBM_OK;
+}
+
+/* ---- single-cell entry points for unit tests ---- */
+
+#define ORC_CELL_CONSTS(T)                                                                  \
+    T w[27]; int crev[27]; T c13;                                                           \
+    for (int q = 0; q < 27; ++q) {                                                          \
+        int k = orc_nnz(q);                                                                 \
+        w[q] = (compat & LBM_COMPAT_F32CONST)                                               \
+                   ? (T)(k == 0 ? 8.f / 27.f : k == 1 ? 2.f / 27.f : k == 2 ? 1.f / 54.f : 1.f / 216.f) \
+                   : (T)(k == 0 ? 8.0 / 27.0 : k == 1 ? 2.0 / 27.0 : k == 2 ? 1.0 / 54.0 : 1.0 / 216.0); \
+    }                                                                                       \
+    orc_crev(compat, crev); (void)crev;                                                     \
+    c13 = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);
+
+void orc_tables(uint32_t compat, int* c, int* c_rev, double* wout)
+{
+    ORC_CELL_CONSTS(double)
+    (void)c13;
+    for (int q = 0; q < 27; ++q) {
+        if (c) { c[3 * q] = ORC_C[q][0]; c[3 * q + 1] = ORC_C[q][1]; c[3 * q + 2] = ORC_C[q][2]; }
+        if (c_rev) c_rev[q] = crev[q];
+        if (wout) wout[q] = w[q];
+    }
+}
+
+void orc_cell_collide(int collision, uint32_t compat, int use_double, const double* f, double rho, double ux,
+                      double uy, double uz, double omega, double* out)
+{
+    if (use_double) {
+        ORC_CELL_CONSTS(double)
+        double o[27];
+        if (collision == LBM_COLLISION_CUM2015)
+            cum2015_d(f, rho, ux, uy, uz, omega, c13, o);
+        else
+            srt_d(w, f, rho, ux, uy, uz, omega, o);
+        for (int q = 0; q < 27; ++q) out[q] = o[q];
+    } else {
+        ORC_CELL_CONSTS(float)
+        float fi[27], o[27];
+        for (int q = 0; q < 27; ++q) fi[q] = (float)f[q];
+        if (collision == LBM_COLLISION_CUM2015)
+            cum2015_f(fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)omega, c13, o);
+        else
+            srt_f(w, fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)omega, o);
+        for (int q = 0; q < 27; ++q) out[q] = o[q];
+    }
+}
+
+double orc_cell_omega_les(uint32_t compat, int use_double, const double* f, double rho, double ux, double uy,
+                          double uz, double tau, double c_les)
+{
+    if (use_double) {
+        ORC_CELL_CONSTS(double)
+        (void)c13;
+        return omega_les_d(w, f, rho, ux, uy, uz, tau, c_les);
+    } else {
+        ORC_CELL_CONSTS(float)
+        (void)c13;
+        float fi[27];
+        for (int q = 0; q < 27; ++q) fi[q] = (float)f[q];
+        return omega_les_f(w, fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)tau, (float)c_les);
+    }
+}
+
+void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double uy, double uz, double* out)
+{
+    if (use_double) {
+        ORC_CELL_CONSTS(double)
+        (void)c13;
+        for (int q = 0; q < 27; ++q) out[q] = feq_d(w, q, ux, uy, uz, rho);
+    } else {
+        ORC_CELL_CONSTS(float)
+        (void)c13;
+        for (int q = 0; q < 27; ++q) out[q] = feq_f(w, q, (float)ux, (float)uy, (float)uz, (float)rho);
+    }
+}

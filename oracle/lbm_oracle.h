@@ -1,0 +1,61 @@
+/*
+ * lbm_oracle.h -- CPU oracle of the reference collide-stream path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of stloufra/LB's
+ * unfused two-lattice ("AB") time step, used as the checker for the HIP core
+ * and as the CPU baseline of bench.py.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product never does.
+ *
+ * It keeps the reference's arrays (df, df_post, rho, u, omega, flags, BC
+ * lists), its operator order (SolverTurbulentLES.h:184-214) and the operation
+ * order of every arithmetic expression, so that an FP64 run reproduces the
+ * reference's FP64 fields.  Layout: df[q][z][y][x], x fastest.
+ *
+ * Extensions beyond the reference, each documented where implemented:
+ *   - index-wrap periodicity (desc.periodic), equivalent to the reference's
+ *     ghost-layer Periodic policy for interior cells;
+ *   - corrected tables/literals when desc.compat clears the quirk bits.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/lbm_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+/* Same descriptor as the HIP core.  storage F64 -> double arithmetic,
+ * anything else -> float arithmetic (the reference's RealType switch,
+ * traits/LBMTraits.h:72).  Returns an lbm_status code. */
+int orc_create(const lbm_desc* d, orc** out);
+int orc_init_equilibrium(orc* o, const double* rho, const double* ux, const double* uy, const double* uz);
+int orc_set_populations(orc* o, const double* f);
+int orc_step(orc* o, int nsteps);
+int orc_get_macroscopic(orc* o, double* rho, double* ux, double* uy, double* uz);
+int orc_get_populations(orc* o, double* f);
+int orc_get_post_populations(orc* o, double* f);
+int orc_get_omega(orc* o, double* omega);
+/* ErrorQuadratic::errorEvaluation (errorEvaluations/ErrorQuadratic.h:33-90). */
+int orc_error_quadratic(orc* o, double* err);
+void orc_destroy(orc* o);
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* Single-cell operators, for unit tests against closed forms.
+ * f and out hold 27 values in reference direction order. */
+void orc_cell_collide(int collision, uint32_t compat, int use_double, const double* f, double rho,
+                      double ux, double uy, double uz, double omega, double* out);
+double orc_cell_omega_les(uint32_t compat, int use_double, const double* f, double rho, double ux,
+                          double uy, double uz, double tau, double c_les);
+void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double uy, double uz, double* out);
+void orc_tables(uint32_t compat, int* c /*27*3*/, int* c_rev /*27*/, double* w /*27*/);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,539 @@
+/*
+ * lbm_oracle_impl.h -- precision-generic body of the CPU oracle.
+ * TEST INFRASTRUCTURE ONLY (see lbm_oracle.h).
+ *
+ * Included twice by lbm_oracle.c with REAL = double / float and FN(x) giving
+ * the suffixed name.  Every expression keeps the operand order of the
+ * reference line it cites, and float literals stay float literals, so that
+ * C's promotion rules reproduce the reference's mixed float/RealType
+ * arithmetic (quirk Q2).  Build with -ffp-contract=off.
+ */
+
+typedef struct FN(state) {
+    int nx, ny, nz;
+    long N;
+    int per[3];
+    int collision, closure;
+    uint32_t compat;
+    REAL tau, nu, CLES, c13, cs;
+    REAL w[27];
+    int crev[27];
+    int* flags;
+    REAL* df;       /* [q][N] */
+    REAL* df_post;  /* [q][N] */
+    REAL* rho;      /* [N] */
+    REAL* u;        /* [3N], AoS as the reference's StaticVector field */
+    REAL* omega;    /* [N] */
+    REAL* u_err;    /* [3N] previous velocity for ErrorQuadratic */
+    lbm_bc_node* bc;
+    int n_bc;
+} FN(state);
+
+/* f_equilibrium (OmegaLES.h:47-66, CollisionSRTTurbulent.h:35-54,
+ * InitializationEquilibriumConstVector.h:53-71, InletVelocityEquilibrium.h:34-49):
+ *   uc = c0*ux + c1*uy + c2*uz ; u2 = ux*ux + uy*uy + uz*uz
+ *   w * rho * (1.f + 3.f*uc + 4.5f*uc*uc - 1.5f*u2)                               */
+static inline REAL FN(feq)(const REAL* w, int q, REAL ux, REAL uy, REAL uz, REAL rho)
+{
+    REAL uc = ORC_C[q][0] * ux + ORC_C[q][1] * uy + ORC_C[q][2] * uz;
+    REAL u2 = ux * ux + uy * uy + uz * uz;
+    return w[q] * rho * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * u2);
+}
+
+/* One step of the Geier-2015 central-moment transform along one axis for the
+ * (m, z, p) triple of a line (CollisionCumD3Q27Turbulent2015.h:86-114 z-pass,
+ * :118-146 y-pass, :150-178 x-pass; all three passes use this form). */
+static inline void FN(fwd)(REAL m, REAL z, REAL p, REAL u, REAL* o0, REAL* o1, REAL* o2)
+{
+    REAL k0 = (p + m) + z;
+    *o0 = k0;
+    *o1 = (p - m) - u * k0;
+    *o2 = (p + m) - 2.f * u * (p - m) + u * u * k0;
+}
+
+/* Inverse transform of one (order 0,1,2) triple back to (z, m, p)
+ * (CollisionCumD3Q27Turbulent2015.h:389-419 x-pass, :422-452 y-pass,
+ * :455-485 z-pass). */
+static inline void FN(inv)(REAL a0, REAL a1, REAL a2, REAL u, REAL* vz, REAL* vm, REAL* vp)
+{
+    *vz = a0 * (1.f - u * u) - 2.f * u * a1 - a2;
+    *vm = (a0 * (u * u - u) + a1 * (2.f * u - 1.f) + a2) * 0.5f;
+    *vp = (a0 * (u * u + u) + a1 * (2.f * u + 1.f) + a2) * 0.5f;
+}
+
+/* Cumulant collision of one cell, literal Geier 2015 as configured in the
+ * reference: CollisionCumD3Q27Turbulent2015.h:32-516 (omega1 = local omega,
+ * omega2..omega10 = 1, :240-249).  f/out in reference direction order. */
+static void FN(cum2015)(const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL omg, REAL c13,
+                        REAL* out)
+{
+    REAL F[3][3][3], A[3][3][3], B[3][3][3], k[3][3][3];
+    int a, b, r, q;
+    for (q = 0; q < 27; ++q) F[ORC_C[q][0] + 1][ORC_C[q][1] + 1][ORC_C[q][2] + 1] = f[q];
+
+    /* central moments: z, then y, then x pass (Eq. G2015 43-45) */
+    for (a = 0; a < 3; ++a)
+        for (b = 0; b < 3; ++b)
+            FN(fwd)(F[a][b][0], F[a][b][1], F[a][b][2], uz, &A[a][b][0], &A[a][b][1], &A[a][b][2]);
+    for (a = 0; a < 3; ++a)
+        for (r = 0; r < 3; ++r)
+            FN(fwd)(A[a][0][r], A[a][1][r], A[a][2][r], uy, &B[a][0][r], &B[a][1][r], &B[a][2][r]);
+    for (b = 0; b < 3; ++b)
+        for (r = 0; r < 3; ++r)
+            FN(fwd)(B[0][b][r], B[1][b][r], B[2][b][r], ux, &k[0][b][r], &k[1][b][r], &k[2][b][r]);
+
+#define K(i, j, l) k[i][j][l]
+    const REAL k_000 = K(0,0,0);
+    /* cumulants, Eq. G2015 47-54 (:188-232) */
+    const REAL C_110 = K(1,1,0), C_101 = K(1,0,1), C_011 = K(0,1,1);
+    const REAL C_200 = K(2,0,0), C_020 = K(0,2,0), C_002 = K(0,0,2);
+    const REAL C_120 = K(1,2,0), C_012 = K(0,1,2), C_201 = K(2,0,1);
+    const REAL C_210 = K(2,1,0), C_021 = K(0,2,1), C_102 = K(1,0,2);
+    const REAL C_111 = K(1,1,1);
+    const REAL C_211 = K(2,1,1) - (K(2,0,0) * K(0,1,1) + 2.f * K(1,0,1) * K(1,1,0)) / rho;
+    const REAL C_121 = K(1,2,1) - (K(0,2,0) * K(1,0,1) + 2.f * K(1,1,0) * K(0,1,1)) / rho;
+    const REAL C_112 = K(1,1,2) - (K(0,0,2) * K(1,1,0) + 2.f * K(0,1,1) * K(1,0,1)) / rho;
+    const REAL C_220 = K(2,2,0) - (K(0,2,0) * K(2,0,0) + 2.f * K(1,1,0) * K(1,1,0)) / rho;
+    const REAL C_022 = K(0,2,2) - (K(0,0,2) * K(0,2,0) + 2.f * K(0,1,1) * K(0,1,1)) / rho;
+    const REAL C_202 = K(2,0,2) - (K(2,0,0) * K(0,0,2) + 2.f * K(1,0,1) * K(1,0,1)) / rho;
+    const REAL C_122 = K(1,2,2) - (K(0,2,0) * K(1,0,2) + K(0,0,2) * K(1,2,0) + 4.f * K(0,1,1) * K(1,1,1) +
+                                   2.f * (K(1,1,0) * K(0,1,2) + K(1,0,1) * K(0,2,1))) / rho;
+    const REAL C_212 = K(2,1,2) - (K(0,0,2) * K(2,1,0) + K(2,0,0) * K(0,1,2) + 4.f * K(1,0,1) * K(1,1,1) +
+                                   2.f * (K(0,1,1) * K(2,0,1) + K(1,1,0) * K(1,0,2))) / rho;
+    const REAL C_221 = K(2,2,1) - (K(2,0,0) * K(0,2,1) + K(0,2,0) * K(2,0,1) + 4.f * K(1,1,0) * K(1,1,1) +
+                                   2.f * (K(1,0,1) * K(1,2,0) + K(0,1,1) * K(2,1,0))) / rho;
+    const REAL C_222 = K(2,2,2) - (4.f * K(1,1,1) * K(1,1,1) + K(2,0,0) * K(0,2,2) + K(0,2,0) * K(2,0,2) +
+                                   K(0,0,2) * K(2,2,0) +
+                                   4.f * (K(0,1,1) * K(2,1,1) + K(1,0,1) * K(1,2,1) + K(1,1,0) * K(1,1,2)) +
+                                   2.f * (K(1,2,0) * K(1,0,2) + K(2,1,0) * K(0,1,2) + K(2,0,1) * K(0,2,1))) / rho +
+                       (16.f * K(1,1,0) * K(1,0,1) * K(0,1,1) +
+                        4.f * (K(1,0,1) * K(1,0,1) * K(0,2,0) + K(0,1,1) * K(0,1,1) * K(2,0,0) +
+                               K(1,1,0) * K(1,1,0) * K(0,0,2)) +
+                        2.f * K(2,0,0) * K(0,2,0) * K(0,0,2)) / rho / rho;
+
+    /* relaxation rates (:240-249) */
+    const REAL omega1 = omg;
+    const REAL omega2 = 1.f, omega3 = 1.f, omega4 = 1.f, omega5 = 1.f, omega6 = 1.f;
+    const REAL omega7 = 1.f, omega8 = 1.f, omega9 = 1.f, omega10 = 1.f;
+
+    /* Eq. G2015 58-60 (:252-259) */
+    const REAL Dxu = -omega1 * 0.5f / rho * (2.f * C_200 - C_020 - C_002) -
+                     omega2 * 0.5f / rho * (C_200 + C_020 + C_002 - k_000);
+    const REAL Dyv = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_020);
+    const REAL Dzw = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_002);
+
+    /* Eq. G2015 55-57, 61-73 (:264-316) */
+    const REAL Cs_110 = (1.f - omega1) * C_110;
+    const REAL Cs_101 = (1.f - omega1) * C_101;
+    const REAL Cs_011 = (1.f - omega1) * C_011;
+    const REAL Eq61 = (1.f - omega1) * (C_200 - C_020) -
+                      3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uy * uy * Dyv);
+    const REAL Eq64 = (1.f - omega1) * (C_200 - C_002) -
+                      3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uz * uz * Dzw);
+    const REAL Eq65 = k_000 * omega2 + (1.f - omega2) * (C_200 + C_020 + C_002) -
+                      3.f * rho * (1.f - omega2 / 2.f) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
+    const REAL Cs_200 = c13 * (Eq61 + Eq64 + Eq65);
+    const REAL Cs_020 = c13 * (Eq64 - 2.f * Eq61 + Eq65);
+    const REAL Cs_002 = c13 * (Eq61 - 2.f * Eq64 + Eq65);
+    const REAL Cs_120 = (-C_102 - C_120) * omega3 * 0.5f + (C_102 - C_120) * omega4 * 0.5f + C_120;
+    const REAL Cs_102 = (-C_102 - C_120) * omega3 * 0.5f + (-C_102 + C_120) * omega4 * 0.5f + C_102;
+    const REAL Cs_210 = (-C_012 - C_210) * omega3 * 0.5f + (C_012 - C_210) * omega4 * 0.5f + C_210;
+    const REAL Cs_012 = (-C_012 - C_210) * omega3 * 0.5f + (-C_012 + C_210) * omega4 * 0.5f + C_012;
+    const REAL Cs_021 = (-C_021 - C_201) * omega3 * 0.5f + (-C_021 + C_201) * omega4 * 0.5f + C_021;
+    const REAL Cs_201 = (-C_021 - C_201) * omega3 * 0.5f + (C_021 - C_201) * omega4 * 0.5f + C_201;
+    const REAL Cs_111 = (1.f - omega5) * C_111;
+    const REAL Eq71 = (1.f - omega6) * (C_220 - 2.f * C_202 + C_022);
+    const REAL Eq72 = (1.f - omega6) * (C_220 + C_202 - 2.f * C_022);
+    const REAL Eq73 = (1.f - omega7) * (C_220 + C_202 + C_022);
+    const REAL Cs_220 = c13 * (Eq71 + Eq72 + Eq73);
+    const REAL Cs_202 = c13 * (-Eq71 + Eq73);
+    const REAL Cs_022 = c13 * (-Eq72 + Eq73);
+    const REAL Cs_211 = (1.f - omega8) * C_211;
+    const REAL Cs_121 = (1.f - omega8) * C_121;
+    const REAL Cs_112 = (1.f - omega8) * C_112;
+    const REAL Cs_221 = (1.f - omega9) * C_221;
+    const REAL Cs_212 = (1.f - omega9) * C_212;
+    const REAL Cs_122 = (1.f - omega9) * C_122;
+    const REAL Cs_222 = (1.f - omega10) * C_222;
+#undef K
+
+    /* back to central moments, Eq. G2015 81-87 (:322-380) */
+    REAL s[3][3][3];
+#define S(i, j, l) s[i][j][l]
+    S(0,0,0) = k_000;
+    S(1,1,0) = Cs_110; S(1,0,1) = Cs_101; S(0,1,1) = Cs_011;
+    S(2,0,0) = Cs_200; S(0,2,0) = Cs_020; S(0,0,2) = Cs_002;
+    S(1,2,0) = Cs_120; S(0,1,2) = Cs_012; S(2,0,1) = Cs_201;
+    S(2,1,0) = Cs_210; S(0,2,1) = Cs_021; S(1,0,2) = Cs_102;
+    S(1,1,1) = Cs_111;
+    S(1,0,0) = -k[1][0][0]; S(0,1,0) = -k[0][1][0]; S(0,0,1) = -k[0][0][1];
+    S(2,1,1) = Cs_211 + (S(2,0,0) * S(0,1,1) + 2.f * S(1,0,1) * S(1,1,0)) / rho;
+    S(1,2,1) = Cs_121 + (S(0,2,0) * S(1,0,1) + 2.f * S(1,1,0) * S(0,1,1)) / rho;
+    S(1,1,2) = Cs_112 + (S(0,0,2) * S(1,1,0) + 2.f * S(0,1,1) * S(1,0,1)) / rho;
+    S(2,2,0) = Cs_220 + (S(0,2,0) * S(2,0,0) + 2.f * S(1,1,0) * S(1,1,0)) / rho;
+    S(0,2,2) = Cs_022 + (S(0,0,2) * S(0,2,0) + 2.f * S(0,1,1) * S(0,1,1)) / rho;
+    S(2,0,2) = Cs_202 + (S(2,0,0) * S(0,0,2) + 2.f * S(1,0,1) * S(1,0,1)) / rho;
+    S(1,2,2) = Cs_122 + (S(0,2,0) * S(1,0,2) + S(0,0,2) * S(1,2,0) + 4.f * S(0,1,1) * S(1,1,1) +
+                         2.f * (S(1,1,0) * S(0,1,2) + S(1,0,1) * S(0,2,1))) / rho;
+    S(2,1,2) = Cs_212 + (S(0,0,2) * S(2,1,0) + S(2,0,0) * S(0,1,2) + 4.f * S(1,0,1) * S(1,1,1) +
+                         2.f * (S(0,1,1) * S(2,0,1) + S(1,1,0) * S(1,0,2))) / rho;
+    S(2,2,1) = Cs_221 + (S(2,0,0) * S(0,2,1) + S(0,2,0) * S(2,0,1) + 4.f * S(1,1,0) * S(1,1,1) +
+                         2.f * (S(1,0,1) * S(1,2,0) + S(0,1,1) * S(2,1,0))) / rho;
+    S(2,2,2) = Cs_222 +
+               (4.f * S(1,1,1) * S(1,1,1) + S(2,0,0) * S(0,2,2) + S(0,2,0) * S(2,0,2) + S(0,0,2) * S(2,2,0) +
+                4.f * (S(0,1,1) * S(2,1,1) + S(1,0,1) * S(1,2,1) + S(1,1,0) * S(1,1,2)) +
+                2.f * (S(1,2,0) * S(1,0,2) + S(2,1,0) * S(0,1,2) + S(2,0,1) * S(0,2,1))) / rho -
+               (16.f * S(1,1,0) * S(1,0,1) * S(0,1,1) +
+                4.f * (S(1,0,1) * S(1,0,1) * S(0,2,0) + S(0,1,1) * S(0,1,1) * S(2,0,0) +
+                       S(1,1,0) * S(1,1,0) * S(0,0,2)) +
+                2.f * S(2,0,0) * S(0,2,0) * S(0,0,2)) / rho / rho;
+#undef S
+
+    /* back to populations: x, y, z pass (Eq. G2015 88-96) */
+    for (b = 0; b < 3; ++b)
+        for (r = 0; r < 3; ++r)
+            FN(inv)(s[0][b][r], s[1][b][r], s[2][b][r], ux, &A[1][b][r], &A[0][b][r], &A[2][b][r]);
+    for (a = 0; a < 3; ++a)
+        for (r = 0; r < 3; ++r)
+            FN(inv)(A[a][0][r], A[a][1][r], A[a][2][r], uy, &B[a][1][r], &B[a][0][r], &B[a][2][r]);
+    for (a = 0; a < 3; ++a)
+        for (b = 0; b < 3; ++b)
+            FN(inv)(B[a][b][0], B[a][b][1], B[a][b][2], uz, &F[a][b][1], &F[a][b][0], &F[a][b][2]);
+    for (q = 0; q < 27; ++q) out[q] = F[ORC_C[q][0] + 1][ORC_C[q][1] + 1][ORC_C[q][2] + 1];
+}
+
+/* BGK: f - (f - f_eq)*omega (CollisionSRTTurbulent.h:56-75, CollisionSRTLaminar.h) */
+static void FN(srt)(const REAL* w, const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL omg, REAL* out)
+{
+    for (int q = 0; q < 27; ++q) out[q] = f[q] - (f[q] - FN(feq)(w, q, ux, uy, uz, rho)) * omg;
+}
+
+/* Smagorinsky relaxation rate (OmegaLES.h:69-131). */
+static REAL FN(omega_les)(const REAL* w, const REAL* f, REAL rho, REAL u_x, REAL u_y, REAL u_z, REAL tau,
+                          REAL CLES)
+{
+    REAL pi[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int vel = 0; vel < 27; vel++) {
+        REAL feq = FN(feq)(w, vel, u_x, u_y, u_z, rho);
+        REAL df = f[vel];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) pi[a][b] += ORC_C[vel][a] * ORC_C[vel][b] * (df - feq);
+    }
+    REAL PI = pi[0][0] * pi[0][0];
+    PI += pi[0][1] * pi[0][1];
+    PI += pi[0][2] * pi[0][2];
+    PI += pi[1][0] * pi[1][0];
+    PI += pi[1][1] * pi[1][1];
+    PI += pi[1][2] * pi[1][2];
+    PI += pi[2][0] * pi[2][0];
+    PI += pi[2][1] * pi[2][1];
+    PI += pi[2][2] * pi[2][2];
+    REAL rho_inv = 1.0f / rho;
+    REAL tau_sq = tau * tau;
+    REAL CLES_term = 18.f * CLES * rho_inv;
+    PI = ORC_SQRT(PI);
+    REAL tauLES = tau * 0.5f + 0.5f * ORC_SQRT(tau_sq + CLES_term * PI);
+    return 1.f / tauLES;
+}
+
+static inline long FN(idx)(const FN(state) * s, int x, int y, int z)
+{
+    return (long)x + (long)s->nx * ((long)y + (long)s->ny * (long)z);
+}
+
+/* Neighbour coordinate with optional index wrap; returns 0 when outside the
+ * array along a non-wrapped axis. */
+static inline int FN(nb)(const FN(state) * s, int x, int y, int z, long* out)
+{
+    if (x < 0 || x >= s->nx) { if (!s->per[0]) return 0; x = (x + s->nx) % s->nx; }
+    if (y < 0 || y >= s->ny) { if (!s->per[1]) return 0; y = (y + s->ny) % s->ny; }
+    if (z < 0 || z >= s->nz) { if (!s->per[2]) return 0; z = (z + s->nz) % s->nz; }
+    *out = FN(idx)(s, x, y, z);
+    return 1;
+}
+
+#define DF(s, q, i) ((s)->df[(long)(q) * (s)->N + (i)])
+#define DP(s, q, i) ((s)->df_post[(long)(q) * (s)->N + (i)])
+
+static void FN(load)(const FN(state) * s, const REAL* arr, long i, REAL* f)
+{
+    for (int q = 0; q < 27; ++q) f[q] = arr[(long)q * s->N + i];
+}
+
+/* --- operators, one per reference policy --- */
+
+/* OmegaLES::omega / OmegaNo::omega (OmegaLES.h:134-136, OmegaNo.h:34-47) */
+static void FN(op_omega)(FN(state) * s)
+{
+    long N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        if (s->flags[i] == 0) continue;
+        if (s->closure == LBM_CLOSURE_SMAGORINSKY) {
+            REAL f[27];
+            FN(load)(s, s->df, i, f);
+            s->omega[i] = FN(omega_les)(s->w, f, s->rho[i], s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->tau,
+                                        s->CLES);
+        } else {
+            s->omega[i] = 1.f / s->tau;
+        }
+    }
+}
+
+/* Collision*::collision (Cum2015.h:521-536, CollisionSRTTurbulent.h:56-80) */
+static void FN(op_collision)(FN(state) * s)
+{
+    long N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        if (s->flags[i] == 0) continue;
+        REAL f[27], o[27];
+        FN(load)(s, s->df, i, f);
+        if (s->collision == LBM_COLLISION_CUM2015)
+            FN(cum2015)(f, s->rho[i], s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->omega[i], s->c13, o);
+        else
+            FN(srt)(s->w, f, s->rho[i], s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->omega[i], o);
+        for (int q = 0; q < 27; ++q) DP(s, q, i) = o[q];
+    }
+}
+
+/* StreamingAB::streaming (StreamingAB.h:38-68): pull from x - c_i. */
+static void FN(op_streaming)(FN(state) * s)
+{
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < s->nz; ++z)
+        for (int y = 0; y < s->ny; ++y)
+            for (int x = 0; x < s->nx; ++x) {
+                long i = FN(idx)(s, x, y, z);
+                int mv = s->flags[i];
+                if (!(mv > 0 || mv == -2 || mv == -3)) continue;
+                for (int vel = 0; vel < 27; ++vel) {
+                    long j;
+                    if (!FN(nb)(s, x - ORC_C[vel][0], y - ORC_C[vel][1], z - ORC_C[vel][2], &j)) continue;
+                    int nv = s->flags[j];
+                    if (nv > 0 || nv == -2 || nv == -3)
+                        DF(s, vel, i) = DP(s, vel, j);
+                    else if (nv == -1)
+                        DF(s, vel, i) = DF(s, vel, j);
+                }
+            }
+}
+
+/* BounceBackWallHalf::bounceBackWall (BounceBackWallHalf.h:29-52) */
+static void FN(op_bounce_back)(FN(state) * s)
+{
+    for (int n = 0; n < s->n_bc; ++n) {
+        const lbm_bc_node* b = &s->bc[n];
+        if (b->kind != LBM_BC_WALL) continue;
+        long i = FN(idx)(s, b->x, b->y, b->z);
+        for (int vel = 0; vel < 27; ++vel) {
+            long j;
+            if (!FN(nb)(s, b->x - ORC_C[vel][0], b->y - ORC_C[vel][1], b->z - ORC_C[vel][2], &j)) continue;
+            if (s->flags[j] == 0) DF(s, vel, i) = DP(s, s->crev[vel], i);
+        }
+    }
+}
+
+/* InletVelocityMovingWall::inlet (InletVelocityMovingWall.h:31-53) and
+ * InletVelocityEquilibrium::inlet (InletVelocityEquilibrium.h:51-103). */
+static void FN(op_inlet)(FN(state) * s)
+{
+    for (int n = 0; n < s->n_bc; ++n) {
+        const lbm_bc_node* b = &s->bc[n];
+        long i = FN(idx)(s, b->x, b->y, b->z);
+        if (b->kind == LBM_BC_INLET_MOVING_WALL) {
+            for (int vel = 0; vel < 27; ++vel) {
+                if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] > 0) {
+                    DF(s, s->crev[vel], i) =
+                        DP(s, vel, i) - 6 * s->w[vel] * s->rho[i] *
+                                            (ORC_C[vel][0] * b->ux + ORC_C[vel][1] * b->uy + ORC_C[vel][2] * b->uz);
+                }
+            }
+        } else if (b->kind == LBM_BC_INLET_EQUILIBRIUM) {
+            const REAL uxb = b->ux, uyb = b->uy, uzb = b->uz;
+            long i1, i2;
+            if (b->nx == 1.0f && b->ny == 0.0f && b->nz == 0.0f) {
+                if (FN(nb)(s, b->x - 1, b->y, b->z, &i1) && FN(nb)(s, b->x - 2, b->y, b->z, &i2)) {
+                    REAL ux1 = s->u[3 * i1], ux2 = s->u[3 * i2];
+                    REAL duxb = (uxb - ux1);
+                    REAL dux1 = (ux1 - ux2);
+                    REAL rhob = s->rho[i1] + s->nu / 3 * (dux1 - duxb);
+                    for (int v = 0; v < 27; ++v) DF(s, v, i) = FN(feq)(s->w, v, uxb, uyb, uzb, rhob);
+                }
+            }
+            if (b->nx == -1.0f && b->ny == 0.0f && b->nz == 0.0f) {
+                if (FN(nb)(s, b->x + 1, b->y, b->z, &i1) && FN(nb)(s, b->x + 2, b->y, b->z, &i2)) {
+                    REAL ux1 = s->u[3 * i1], ux2 = s->u[3 * i2];
+                    REAL duxb = (ux1 - uxb);
+                    REAL dux1 = (ux2 - ux1);
+                    REAL rhob = s->rho[i1] + s->nu / 3 * (duxb - dux1);
+                    for (int v = 0; v < 27; ++v) DF(s, v, i) = FN(feq)(s->w, v, uxb, uyb, uzb, rhob);
+                }
+            }
+        }
+    }
+}
+
+/* OutletDensityInterpolated::outlet (OutletDensityInterpolated.h:170-252) */
+static void FN(op_outlet)(FN(state) * s)
+{
+    for (int n = 0; n < s->n_bc; ++n) {
+        const lbm_bc_node* b = &s->bc[n];
+        if (b->kind != LBM_BC_OUTLET_INTERPOLATED) continue;
+        const REAL density = b->rho;
+        long i = FN(idx)(s, b->x, b->y, b->z);
+        int xp = (b->nx == 1.0f && b->ny == 0.0f && b->nz == 0.0f);
+        int xm = (b->nx == -1.0f && b->ny == 0.0f && b->nz == 0.0f);
+        if (!xp && !xm) continue;
+        if (xp && !b->regular) {
+            for (int vel = 0; vel < 27; ++vel) {
+                long j;
+                if (!FN(nb)(s, b->x, b->y - ORC_C[vel][1], b->z - ORC_C[vel][2], &j)) continue;
+                if (s->flags[j] == 0) DF(s, vel, i) = DF(s, s->crev[vel], i);
+            }
+        }
+        long iu;
+        if (!FN(nb)(s, b->x + (xp ? -1 : 1), b->y, b->z, &iu)) continue;
+        for (int vel = 0; vel < 27; ++vel) {
+            if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] < 0)
+                DF(s, vel, i) = s->cs * DP(s, vel, iu) + (1 - s->cs) * DP(s, vel, i);
+        }
+        REAL d[27];
+        FN(load)(s, s->df, i, d);
+        REAL rho = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8] + d[9] + d[10] + d[11] + d[12] +
+                   d[13] + d[14] + d[15] + d[16] + d[17] + d[18] + d[19] + d[20] + d[21] + d[22] + d[23] + d[24] +
+                   d[25] + d[26];
+        REAL ux = (d[1] - d[2] + d[7] - d[8] + d[9] - d[10] - d[11] + d[12] - d[15] + d[16] - d[19] + d[20] -
+                   d[21] + d[22] + d[23] - d[24] - d[25] + d[26]) / density;
+        REAL uy = (-d[5] + d[6] - d[11] + d[12] + d[13] - d[14] + d[15] - d[16] + d[17] - d[18] + d[19] -
+                   d[20] - d[21] + d[22] - d[23] + d[24] - d[25] + d[26]) / density;
+        REAL uz = (-d[3] + d[4] - d[7] + d[8] + d[9] - d[10] - d[13] + d[14] + d[17] - d[18] - d[19] + d[20] +
+                   d[21] - d[22] - d[23] + d[24] - d[25] + d[26]) / density;
+        for (int vel = 0; vel < 27; ++vel)
+            DF(s, vel, i) = DF(s, vel, i) - FN(feq)(s->w, vel, ux, uy, uz, rho) + FN(feq)(s->w, vel, ux, uy, uz, 1.f);
+    }
+}
+
+/* Periodic::periodic (Periodic.h:43-138), ghost-layer periodicity. */
+static void FN(op_periodic)(FN(state) * s)
+{
+    for (int n = 0; n < s->n_bc; ++n) {
+        const lbm_bc_node* b = &s->bc[n];
+        if (b->kind != LBM_BC_PERIODIC_GHOST) continue;
+        long i = FN(idx)(s, b->x, b->y, b->z);
+        float ax = fabsf(b->nx), ay = fabsf(b->ny), az = fabsf(b->nz);
+        int axis = (ax == 1.0f && ay == 0.0f && az == 0.0f) ? 0
+                 : (ax == 0.0f && ay == 1.0f && az == 0.0f) ? 1
+                 : (ax == 0.0f && ay == 0.0f && az == 1.0f) ? 2 : -1;
+        if (axis < 0) continue;
+        long ip = axis == 0 ? FN(idx)(s, b->per_idx, b->y, b->z)
+                : axis == 1 ? FN(idx)(s, b->x, b->per_idx, b->z)
+                            : FN(idx)(s, b->x, b->y, b->per_idx);
+        for (int vel = 0; vel < 27; ++vel) {
+            float nc = b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2];
+            if (nc < 0) DF(s, vel, i) = DF(s, vel, ip);
+            if (!b->regular && nc <= 0) {
+                long j;
+                if (FN(nb)(s, b->x + ORC_C[vel][0], b->y + ORC_C[vel][1], b->z + ORC_C[vel][2], &j) &&
+                    s->flags[j] == 0)
+                    DF(s, s->crev[vel], i) = DP(s, vel, i);
+            }
+        }
+    }
+}
+
+/* MomentDensityVelocityN27::momentUpdate (MomentDensityVelocityN27.h:27-83) */
+static void FN(moments_cell)(const REAL* d, REAL* rho, REAL* ux, REAL* uy, REAL* uz)
+{
+    REAL r = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8] + d[9] + d[10] + d[11] + d[12] + d[13] +
+             d[14] + d[15] + d[16] + d[17] + d[18] + d[19] + d[20] + d[21] + d[22] + d[23] + d[24] + d[25] + d[26];
+    *rho = r;
+    *ux = (d[1] - d[2] + d[7] - d[8] + d[9] - d[10] - d[11] + d[12] - d[15] + d[16] - d[19] + d[20] - d[21] +
+           d[22] + d[23] - d[24] - d[25] + d[26]) / r;
+    *uy = (-d[5] + d[6] - d[11] + d[12] + d[13] - d[14] + d[15] - d[16] + d[17] - d[18] + d[19] - d[20] - d[21] +
+           d[22] - d[23] + d[24] - d[25] + d[26]) / r;
+    *uz = (-d[3] + d[4] - d[7] + d[8] + d[9] - d[10] - d[13] + d[14] + d[17] - d[18] - d[19] + d[20] + d[21] -
+           d[22] - d[23] + d[24] - d[25] + d[26]) / r;
+}
+
+static void FN(op_moments)(FN(state) * s)
+{
+    long N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        if (s->flags[i] == 0) continue;
+        REAL d[27];
+        FN(load)(s, s->df, i, d);
+        FN(moments_cell)(d, &s->rho[i], &s->u[3 * i], &s->u[3 * i + 1], &s->u[3 * i + 2]);
+    }
+}
+
+/* SolverTurbulentLES::runSimulation body (SolverTurbulentLES.h:187-214). */
+static void FN(step)(FN(state) * s)
+{
+    FN(op_omega)(s);
+    FN(op_collision)(s);
+    FN(op_streaming)(s);
+    FN(op_bounce_back)(s);
+    FN(op_inlet)(s);
+    FN(op_outlet)(s);
+    FN(op_periodic)(s);
+    FN(op_moments)(s);
+}
+
+/* InitializationEquilibriumConstVector (:74-126), with per-cell rho/u. */
+static void FN(init_eq)(FN(state) * s, const double* rho, const double* ux, const double* uy, const double* uz,
+                        double rho0)
+{
+    long N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        if (s->flags[i] != 0) {
+            s->rho[i] = rho ? (REAL)rho[i] : (REAL)rho0;
+            s->u[3 * i] = ux ? (REAL)ux[i] : 0;
+            s->u[3 * i + 1] = uy ? (REAL)uy[i] : 0;
+            s->u[3 * i + 2] = uz ? (REAL)uz[i] : 0;
+        } else {
+            s->rho[i] = 0.f;
+            s->u[3 * i] = s->u[3 * i + 1] = s->u[3 * i + 2] = 0.f;
+        }
+        s->u_err[3 * i] = s->u_err[3 * i + 1] = s->u_err[3 * i + 2] = 0;
+        for (int vel = 0; vel < 27; ++vel) {
+            REAL v = 0.f;
+            if (s->flags[i] != 0) v = FN(feq)(s->w, vel, s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->rho[i]);
+            DF(s, vel, i) = v;
+            DP(s, vel, i) = v;
+        }
+    }
+}
+
+static void FN(set_pop)(FN(state) * s, const double* f)
+{
+    long N = s->N;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i)
+        for (int q = 0; q < 27; ++q) DF(s, q, i) = s->flags[i] != 0 ? (REAL)f[(long)q * N + i] : 0;
+    FN(op_moments)(s);
+}
+
+/* ErrorQuadratic::errorEvaluation (ErrorQuadratic.h:33-90), including the
+ * y-z component typo of :43-44.  The TNL reduce order is not reproduced
+ * (summation here is sequential in double). */
+static double FN(error_quadratic)(FN(state) * s)
+{
+    double e1 = 0.0, e2 = 0.0;
+    for (long i = 0; i < s->N; ++i) {
+        if (s->flags[i] != 1) continue;
+        const REAL* u = &s->u[3 * i];
+        const REAL* e = &s->u_err[3 * i];
+        REAL add1 = (u[0] - e[0]) * (u[0] - e[0]) + (u[1] - e[1]) * (u[1] - e[1]) + (u[1] - e[2]) * (u[1] - e[2]);
+        REAL add2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        e1 += ORC_SQRT(add1);
+        e2 += ORC_SQRT(add2);
+    }
+    for (long i = 0; i < 3 * s->N; ++i) s->u_err[i] = s->u[i];
+    return sqrt(e1) / sqrt(e2);
+}
+
+#undef DF
+#undef DP

@@ -1,0 +1,85 @@
+// Host build of the core's per-cell collision header (lb_amd/csrc/collide.hpp)
+// for the CPU test suite: checks the product arithmetic against the oracle
+// without a GPU.  Built with -ffp-contract=off, like the oracle.
+#include <cstdint>
+
+#include "collide.hpp"
+
+using namespace lbm;
+
+template <class R>
+static Consts<R> consts(uint32_t compat)
+{
+    Consts<R> K;
+    if (compat & 2u) {
+        K.w[0] = (R)(8.f / 27.f); K.w[1] = (R)(2.f / 27.f); K.w[2] = (R)(1.f / 54.f); K.w[3] = (R)(1.f / 216.f);
+        K.c13 = (R)(1.f / 3.f);
+    } else {
+        K.w[0] = (R)(8.0 / 27.0); K.w[1] = (R)(2.0 / 27.0); K.w[2] = (R)(1.0 / 54.0); K.w[3] = (R)(1.0 / 216.0);
+        K.c13 = (R)(1.0 / 3.0);
+    }
+    return K;
+}
+
+template <class R>
+static void run(int coll, int fast, uint32_t compat, const double* fin, double rho, double ux, double uy, double uz,
+                double om, double* out)
+{
+    const Consts<R> K = consts<R>(compat);
+    R f[Q];
+    for (int q = 0; q < Q; ++q) f[q] = (R)fin[q];
+    if (!fast) {
+        if (coll == 1) cum2015_literal(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+        else srt_literal(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+    } else {
+        const Raw2<R> r = raw_moments(f);
+        const Cen2<R> c = central2(r, (R)ux, (R)uy, (R)uz);
+        if (coll == 1) cum2015_fast(K, c, r.m0, (R)rho, (R)ux, (R)uy, (R)uz, (R)om, f);
+        else srt_fast(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+    }
+    for (int q = 0; q < Q; ++q) out[q] = (double)f[q];
+}
+
+extern "C" {
+
+void host_collide(int coll, int fast, int use_double, uint32_t compat, const double* f, double rho, double ux,
+                  double uy, double uz, double om, double* out)
+{
+    if (use_double) run<double>(coll, fast, compat, f, rho, ux, uy, uz, om, out);
+    else run<float>(coll, fast, compat, f, rho, ux, uy, uz, om, out);
+}
+
+void host_moments(int use_double, const double* fin, double* out4)
+{
+    if (use_double) {
+        double f[Q], r, x, y, z;
+        for (int q = 0; q < Q; ++q) f[q] = fin[q];
+        moments_literal(f, r, x, y, z);
+        out4[0] = r; out4[1] = x; out4[2] = y; out4[3] = z;
+    } else {
+        float f[Q], r, x, y, z;
+        for (int q = 0; q < Q; ++q) f[q] = (float)fin[q];
+        moments_literal(f, r, x, y, z);
+        out4[0] = r; out4[1] = x; out4[2] = y; out4[3] = z;
+    }
+}
+
+double host_omega_les(int fast, int use_double, uint32_t compat, const double* fin, double rho, double ux, double uy,
+                      double uz, double tau, double cles)
+{
+    if (use_double) {
+        const Consts<double> K = consts<double>(compat);
+        double f[Q];
+        for (int q = 0; q < Q; ++q) f[q] = fin[q];
+        if (!fast) return omega_les_literal(K, f, rho, ux, uy, uz, tau, cles);
+        const Raw2<double> r = raw_moments(f);
+        return omega_les_fast(central2(r, ux, uy, uz), rho, tau, cles);
+    }
+    const Consts<float> K = consts<float>(compat);
+    float f[Q];
+    for (int q = 0; q < Q; ++q) f[q] = (float)fin[q];
+    if (!fast) return omega_les_literal(K, f, (float)rho, (float)ux, (float)uy, (float)uz, (float)tau, (float)cles);
+    const Raw2<float> r = raw_moments(f);
+    return omega_les_fast(central2(r, (float)ux, (float)uy, (float)uz), (float)rho, (float)tau, (float)cles);
+}
+}
