@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Headline benchmark: D3Q27 cumulant + Smagorinsky LES, 256^3, FP32, one MI355X.
+
+Workload (config C2): periodic decaying turbulence on 256^3 cells, initial
+velocity from seeded random Fourier modes with E(k) ~ k^4 exp(-2 (k/k0)^2),
+k0 = 4, rms u = 0.04 (seed 20240601), rho = 1, f = f_eq; cumulant-2015
+collision with the Smagorinsky rate (C = 0.17 in the reference formula),
+tau0 = 0.5015 (nu0 = 5e-4).  A step is one fused collide-stream launch over
+all 16,777,216 cells; populations stay resident in HBM.
+
+Prints ONE JSON line.  `value` is MLUPS (million cell updates per second)
+from the wall clock around K steps bracketed by device synchronisation; the
+roofline block uses the HIP-event time of the collide-stream kernel on the
+stream it is launched on.  The CPU baseline is the CPU oracle (a restatement
+of the reference's unfused two-lattice step) on a bounded sample of the same
+workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from lb_amd import Lattice, abi, make_desc  # noqa: E402
+from lb_amd.fields import decaying_turbulence  # noqa: E402
+
+METRIC = "MLUPS + % HBM roofline, D3Q27 cumulant 256³; NN closure vs Smagorinsky err"
+HBM_PEAK_GBS = 8000.0
+SEED = 20240601
+TAU0 = 0.5015          # nu0 = 5e-4
+C_LES = 0.17
+STORAGES = {"f64": abi.STORAGE_F64, "f32": abi.STORAGE_F32, "f16dev": abi.STORAGE_F16DEV,
+            "bf16dev": abi.STORAGE_BF16DEV}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_lattice(n, storage, fields, arith=abi.ARITH_FAST):
+    flags = np.ones((n, n, n), np.int32)
+    d = make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY, storage=storage,
+                  arith=arith, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=TAU0, c_les=C_LES)
+    L = Lattice(d)
+    L.init_equilibrium(None, *fields)
+    return L
+
+
+def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST):
+    L = make_lattice(n, storage, fields, arith)
+    L.step(warmup)
+    L.sync()
+    t0 = time.perf_counter()
+    L.step(steps)
+    L.sync()
+    wall = time.perf_counter() - t0
+    kms = L.last_kernel_ms()
+    mass = L.conservation()
+    L.close()
+    cells = n ** 3
+    return {"wall_s": wall, "ms_per_step": wall / steps * 1e3, "kernel_ms": kms,
+            "mlups": cells * steps / wall / 1e6, "mass": float(mass[0]), "momentum": [float(m) for m in mass[1:]]}
+
+
+def cpu_baseline(n, fields, steps, warmup):
+    """CPU oracle (reference-order AB step, FP32) on the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleLattice, oracle_lib
+
+    flags = np.ones((n, n, n), np.int32)
+    d = make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+                  storage=abi.STORAGE_F32, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE,
+                  periodic=(1, 1, 1), tau=TAU0, c_les=C_LES)
+    o = OracleLattice(d)
+    o.init_equilibrium(None, *fields)
+    o.step(warmup)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    cores = oracle_lib().orc_get_threads()
+    o.close()
+    return {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle (reference unfused AB order: omega, collision, streaming, moments), FP32, "
+                      f"same {n}^3 workload, {warmup} warm-up + {steps} timed steps, {cores} OpenMP threads, "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--storage", default="f32", choices=sorted(STORAGES))
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip the other storage formats")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the collide-stream core runs on a single MI355X; "
+              "multi-GPU slab decomposition is a design, not an implementation) -- run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n_dev, dev = abi.device_info()
+    if n_dev <= 0:
+        raise SystemExit(f"no HIP device visible: {dev}")
+    log(f"device: {dev}")
+    probe_gbs, _ = 0.0, None
+    g = abi.load_core()
+    import ctypes
+    gb, ms = ctypes.c_double(), ctypes.c_double()
+    if g.lbm_bandwidth_probe(1 << 30, 20, ctypes.byref(gb), ctypes.byref(ms)) == abi.LBM_OK:
+        probe_gbs = gb.value
+    log(f"copy-kernel bandwidth: {probe_gbs:.0f} GB/s")
+
+    n = args.n
+    t0 = time.perf_counter()
+    fields = decaying_turbulence(n, k0=4.0, u_rms=0.04, seed=SEED)
+    log(f"initial field {n}^3 generated in {time.perf_counter() - t0:.1f} s")
+
+    storage = STORAGES[args.storage]
+    sbytes = abi.STORAGE_BYTES[storage]
+    res = run_gpu(n, storage, fields, args.steps, args.warmup)
+    log(f"{args.storage}: {res['mlups']:.0f} MLUPS, kernel {res['kernel_ms']:.4f} ms/step, "
+        f"wall {res['ms_per_step']:.4f} ms/step, mass {res['mass']:.6e}")
+    if not np.isfinite(res["mass"]):
+        raise SystemExit("non-finite mass after the timed run")
+
+    cells = n ** 3
+    bytes_per_launch = cells * 2 * 27 * sbytes
+    achieved = bytes_per_launch / (res["kernel_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_step (fused collide-stream, A-A)", "bytes_per_cell": 2 * 27 * sbytes,
+                "kernel_ms": round(res["kernel_ms"], 5), "copy_kernel_gbs": round(probe_gbs, 1)}
+
+    variants = {}
+    if not args.no_variants:
+        vsteps = max(200, args.steps // 5)
+        for name in ("f16dev", "bf16dev", "f64"):
+            if name == args.storage:
+                continue
+            r = run_gpu(n, STORAGES[name], fields, vsteps, 20)
+            sb = abi.STORAGE_BYTES[STORAGES[name]]
+            ach = cells * 54 * sb / (r["kernel_ms"] * 1e-3) / 1e9
+            variants[name] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                              "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                              "steps": vsteps, "bytes_per_cell": 54 * sb}
+            log(f"{name}: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, fields, args.cpu_steps, 2)
+        log(f"cpu baseline: {cpu['value']} MLUPS on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(res["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage == abi.STORAGE_F32 else ("f64" if storage == abi.STORAGE_F64 else "f32"),
+        "data": "synthetic",
+        "config": {"workload": f"C2 decaying turbulence {n}^3 periodic, D3Q27 cumulant-2015 + Smagorinsky "
+                               f"(C=0.17), {args.storage} population storage, fused A-A collide-stream",
+                   "cells": cells, "storage": args.storage, "tau0": TAU0, "c_les": C_LES,
+                   "spectrum": "E(k)~k^4 exp(-2(k/4)^2), u_rms=0.04", "seed": SEED, "device": dev},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

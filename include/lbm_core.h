@@ -154,6 +154,11 @@ lbm_status lbm_step_count(lbm_ctx* ctx, int64_t* steps);
  * call, in milliseconds (HIP events on the context stream). */
 lbm_status lbm_last_kernel_ms(lbm_ctx* ctx, double* ms);
 
+/* Device-to-device copy of `bytes` bytes (a plain float4 stream kernel) run
+ * `iters` times on the current device: the practical HBM bandwidth this
+ * core's roofline fractions are reported next to.  gbps = 2*bytes/time. */
+lbm_status lbm_bandwidth_probe(size_t bytes, int32_t iters, double* gbps, double* ms_per_iter);
+
 /* Bytes of device memory held by the context. */
 lbm_status lbm_device_bytes(lbm_ctx* ctx, size_t* bytes);
 

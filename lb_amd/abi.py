@@ -65,6 +65,7 @@ CORE_SYMBOLS = [
     ("lbm_step_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("lbm_last_kernel_ms", ctypes.c_int, [_vp, _dp]),
     ("lbm_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    ("lbm_bandwidth_probe", ctypes.c_int, [ctypes.c_size_t, ctypes.c_int32, _dp, _dp]),
     ("lbm_device_info", ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]),
     ("lbm_status_str", ctypes.c_char_p, [ctypes.c_int]),
     ("lbm_destroy", None, [_vp]),

@@ -566,6 +566,13 @@ __global__ __launch_bounds__(TPB) void k_sums(AuxArgs<ST> a)
     }
 }
 
+// Plain stream copy used as the bandwidth reference.
+__global__ __launch_bounds__(TPB) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 }  // namespace lbm
 
 // ===========================================================================
@@ -1207,6 +1214,41 @@ lbm_status lbm_device_info(int32_t* count, char* name, size_t name_len)
         std::snprintf(name, name_len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
     }
     return LBM_OK;
+}
+
+lbm_status lbm_bandwidth_probe(size_t bytes, int32_t iters, double* gbps, double* ms_per_iter)
+{
+    if (bytes < 16 || iters <= 0) return LBM_E_ARG;
+    const size_t n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) { (void)hipGetLastError(); return LBM_E_OOM; }
+    if (hipMalloc(&b, n * 16) != hipSuccess) { (void)hipGetLastError(); (void)hipFree(a); return LBM_E_OOM; }
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    lbm_status st = LBM_OK;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { (void)hipFree(a); (void)hipFree(b); return LBM_E_HIP; }
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const unsigned nb = (unsigned)((n + TPB - 1) / TPB);
+    (void)hipMemsetAsync(a, 0, n * 16, s);
+    k_copy<<<nb, TPB, 0, s>>>(a, b, n);  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < iters; ++k) k_copy<<<nb, TPB, 0, s>>>(a, b, n);
+    (void)hipEventRecord(e1, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) st = LBM_E_HIP;
+    float ms = 0.f;
+    if (st == LBM_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = LBM_E_HIP;
+    if (st == LBM_OK) {
+        const double per = (double)ms / iters;
+        if (ms_per_iter) *ms_per_iter = per;
+        if (gbps) *gbps = 2.0 * (double)(n * 16) / (per * 1e-3) / 1e9;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return st;
 }
 
 lbm_status lbm_device_bytes(lbm_ctx* c, size_t* bytes)
