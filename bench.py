@@ -40,6 +40,18 @@ STORAGES = {"f64": abi.STORAGE_F64, "f32": abi.STORAGE_F32, "f16dev": abi.STORAG
             "bf16dev": abi.STORAGE_BF16DEV}
 
 
+def profiled_traffic(storage_name, n):
+    """HBM bytes per launch of this kernel/workload from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel
+    of known size), or None when that workload has not been profiled."""
+    path = os.path.join(ROOT, "profiles", f"k_step_{storage_name}_c2_{n}cube.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d["traffic_bytes_per_launch"]["mean"], os.path.relpath(path, ROOT)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -138,8 +150,10 @@ def main():
     cells = n ** 3
     bytes_per_launch = cells * 2 * 27 * sbytes
     achieved = bytes_per_launch / (res["kernel_ms"] * 1e-3) / 1e9
+    traffic, traffic_src = profiled_traffic(args.storage, n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "algorithmic_bytes_per_launch": bytes_per_launch,
                 "kernel": "k_step (fused collide-stream, A-A)", "bytes_per_cell": 2 * 27 * sbytes,
                 "kernel_ms": round(res["kernel_ms"], 5), "copy_kernel_gbs": round(probe_gbs, 1)}
 

@@ -333,3 +333,33 @@ void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double
         for (int q = 0; q < 27; ++q) out[q] = feq_f(w, q, (float)ux, (float)uy, (float)uz, (float)rho);
     }
 }
+
+void orc_cell_central_moments(const double* f, double ux, double uy, double uz, double* kout)
+{
+    double F[3][3][3], A[3][3][3], B[3][3][3], k[3][3][3];
+    for (int q = 0; q < 27; ++q) F[ORC_C[q][0] + 1][ORC_C[q][1] + 1][ORC_C[q][2] + 1] = f[q];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) fwd_d(F[a][b][0], F[a][b][1], F[a][b][2], uz, &A[a][b][0], &A[a][b][1], &A[a][b][2]);
+    for (int a = 0; a < 3; ++a)
+        for (int r = 0; r < 3; ++r) fwd_d(A[a][0][r], A[a][1][r], A[a][2][r], uy, &B[a][0][r], &B[a][1][r], &B[a][2][r]);
+    for (int b = 0; b < 3; ++b)
+        for (int r = 0; r < 3; ++r) fwd_d(B[0][b][r], B[1][b][r], B[2][b][r], ux, &k[0][b][r], &k[1][b][r], &k[2][b][r]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l) kout[9 * i + 3 * j + l] = k[i][j][l];
+}
+
+void orc_cell_from_central_moments(const double* kin, double ux, double uy, double uz, double* f)
+{
+    double s[3][3][3], A[3][3][3], B[3][3][3], F[3][3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l) s[i][j][l] = kin[9 * i + 3 * j + l];
+    for (int b = 0; b < 3; ++b)
+        for (int r = 0; r < 3; ++r) inv_d(s[0][b][r], s[1][b][r], s[2][b][r], ux, &A[1][b][r], &A[0][b][r], &A[2][b][r]);
+    for (int a = 0; a < 3; ++a)
+        for (int r = 0; r < 3; ++r) inv_d(A[a][0][r], A[a][1][r], A[a][2][r], uy, &B[a][1][r], &B[a][0][r], &B[a][2][r]);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) inv_d(B[a][b][0], B[a][b][1], B[a][b][2], uz, &F[a][b][1], &F[a][b][0], &F[a][b][2]);
+    for (int q = 0; q < 27; ++q) f[q] = F[ORC_C[q][0] + 1][ORC_C[q][1] + 1][ORC_C[q][2] + 1];
+}

@@ -53,6 +53,10 @@ double orc_cell_omega_les(uint32_t compat, int use_double, const double* f, doub
                           double uy, double uz, double tau, double c_les);
 void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double uy, double uz, double* out);
 void orc_tables(uint32_t compat, int* c /*27*3*/, int* c_rev /*27*/, double* w /*27*/);
+/* The three-pass central-moment transform of the cumulant operator and its
+ * inverse (FP64): k[9*i + 3*j + l] = central moment of orders (i, j, l). */
+void orc_cell_central_moments(const double* f, double ux, double uy, double uz, double* k);
+void orc_cell_from_central_moments(const double* k, double ux, double uy, double uz, double* f);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,8 @@ def oracle_lib():
             "orc_cell_feq": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, _dp]),
             "orc_tables": (None, [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]),
+            "orc_cell_central_moments": (None, [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+            "orc_cell_from_central_moments": (None, [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
         }
         for k, (r, a) in sig.items():
             fn = getattr(L, k)

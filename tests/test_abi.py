@@ -1,0 +1,74 @@
+"""The C-ABI libraries load and export every declared symbol; descriptor
+validation runs on the host (no GPU needed for these)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import ROOT, cavity_setup
+from lb_amd import Lattice, LbmError, abi, make_desc
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:lbm|vox|lbmh)_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_core_exports_every_declared_symbol():
+    lib = ctypes.CDLL(abi.CORE_LIB)
+    names = declared("lbm_core.h")
+    assert "lbm_create" in names and "lbm_step" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers the same set
+    assert sorted(n for n, _, _ in abi.CORE_SYMBOLS) == names
+
+
+def test_status_strings():
+    lib = abi.load_core()
+    for s in range(7):
+        assert lib.lbm_status_str(s)
+
+
+def _create_status(desc):
+    lib = abi.load_core()
+    h = ctypes.c_void_p()
+    st = lib.lbm_create(ctypes.byref(desc), ctypes.byref(h))
+    if st == abi.LBM_OK:
+        lib.lbm_destroy(h)
+    return st
+
+
+def test_validation_rejects_bad_descriptors():
+    flags, bc, _ = cavity_setup(4)
+    # tau <= 1/2 (omega >= 2)
+    assert _create_status(make_desc(flags, bc, tau=0.5)) == abi.LBM_E_ARG
+    # fluid cell on a non-periodic array boundary (no halo)
+    assert _create_status(make_desc(np.ones((4, 4, 4), np.int32), [], tau=0.6)) == abi.LBM_E_ARG
+    # inlet flag without an inlet entry
+    f2 = flags.copy()
+    f2[2, 2, 2] = -1
+    assert _create_status(make_desc(f2, bc, tau=0.6)) == abi.LBM_E_ARG
+    # unknown negative id
+    f3 = flags.copy()
+    f3[2, 2, 2] = -7
+    assert _create_status(make_desc(f3, bc, tau=0.6)) == abi.LBM_E_UNSUPPORTED
+    # wall entry on a solid cell
+    bad = list(bc) + [abi.BcNode(0, 0, 0, abi.BC_WALL)]
+    assert _create_status(make_desc(flags, bad, tau=0.6)) == abi.LBM_E_ARG
+    # literal arithmetic is FP64/FP32 only
+    assert _create_status(make_desc(flags, bc, tau=0.6, storage=abi.STORAGE_F16DEV,
+                                    arith=abi.ARITH_LITERAL)) == abi.LBM_E_ARG
+
+
+def test_valid_descriptor_without_device_reports_hip_error():
+    n, _ = abi.device_info()
+    if n > 0:
+        pytest.skip("a GPU is visible")
+    flags, bc, _ = cavity_setup(4)
+    assert _create_status(make_desc(flags, bc, tau=0.6)) == abi.LBM_E_HIP
+    with pytest.raises(LbmError):
+        Lattice(make_desc(flags, bc, tau=0.6))

@@ -1,0 +1,308 @@
+"""Pins of the CPU oracle against what the mathematics fixes (no GPU).
+
+Each test is chosen so that a plausible restatement mistake (a dropped term,
+a wrong sign or index, a transposed operand) fails it: closed forms of the
+D3Q27 lattice, conservation and fixed points of the collision, a brute-force
+definition of the central moments, the viscosity measured from shear-wave
+decay, the equivalence of the reference's ghost-layer periodicity with index
+wrap, and the product collision arithmetic compiled for the host.
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import (OracleLattice, cavity_setup, host_collide_lib, oracle_lib, random_velocity_field, rel_l2,
+                     tables)
+from lb_amd import abi, make_desc
+
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+def feq(rho, u, compat=0):
+    out = np.zeros(27)
+    oracle_lib().orc_cell_feq(compat, 1, rho, *u, abi.ptr(out))
+    return out
+
+
+# --- lattice tables -------------------------------------------------------
+
+def test_velocity_set_and_weights():
+    c, crev, w = tables(0)
+    assert len({tuple(v) for v in c}) == 27 and c.min() == -1 and c.max() == 1
+    assert np.isclose(w.sum(), 1.0, atol=1e-15)
+    assert np.allclose(w @ c, 0.0, atol=1e-16)
+    assert np.allclose(np.einsum("q,qa,qb->ab", w, c, c), np.eye(3) / 3, atol=1e-16)
+    # fourth-order isotropy: sum w c_a c_b c_c c_d = (d_ab d_cd + d_ac d_bd + d_ad d_bc) / 9
+    d = np.eye(3)
+    iso = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 9
+    assert np.allclose(np.einsum("q,qa,qb,qc,qd->abcd", w, c, c, c, c), iso, atol=1e-16)
+    # corrected reverse table is the true opposite
+    assert np.array_equal(c[crev], -c)
+
+
+def test_reference_quirks_q1_q2():
+    c, crev_ref, w_ref = tables(abi.COMPAT_REFERENCE)
+    _, crev, w = tables(0)
+    diff = np.nonzero(crev_ref != crev)[0]
+    assert list(diff) == [20] and crev_ref[20] == 21 and crev[20] == 19
+    # float-rounded weights: equal to the float32 quotient, not the exact value
+    assert w_ref[0] == float(np.float32(8) / np.float32(27)) and w_ref[0] != 8.0 / 27.0
+    assert abs(w_ref.sum() - 1.0) < 1e-7 and w_ref.sum() != 1.0
+
+
+def test_equilibrium_moments_closed_form():
+    c, _, w = tables(0)
+    rho, u = 1.03, np.array([0.04, -0.03, 0.02])
+    f = feq(rho, u)
+    assert np.isclose(f.sum(), rho, rtol=1e-15)
+    assert np.allclose(f @ c, rho * u, atol=1e-16)
+    # second-order polynomial equilibrium of D3Q27: sum c c f_eq = rho (u u + I/3)
+    assert np.allclose(np.einsum("q,qa,qb->ab", f, c, c), rho * (np.outer(u, u) + np.eye(3) / 3), atol=1e-15)
+
+
+# --- collision ------------------------------------------------------------
+
+@pytest.mark.parametrize("coll", [abi.COLLISION_SRT, abi.COLLISION_CUM2015])
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_collision_conserves_mass_and_momentum(coll, dbl):
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(3)
+    O = oracle_lib()
+    for _ in range(20):
+        f = feq(1 + 0.02 * rng.standard_normal(), 0.04 * rng.standard_normal(3)) * (1 + 0.03 * rng.standard_normal(27))
+        if not dbl:
+            f = f.astype(np.float32).astype(np.float64)
+        rho = f.sum()
+        u = f @ c / rho
+        out = np.zeros(27)
+        O.orc_cell_collide(coll, 0, dbl, abi.ptr(f), rho, *u, 1.7, abi.ptr(out))
+        tol = 1e-14 if dbl else 2e-6
+        assert abs(out.sum() - rho) < tol
+        assert np.allclose(out @ c, rho * u, atol=tol)
+
+
+@pytest.mark.parametrize("coll", [abi.COLLISION_SRT, abi.COLLISION_CUM2015])
+def test_rest_equilibrium_is_fixed_point(coll):
+    f = feq(1.0, (0.0, 0.0, 0.0))
+    out = np.zeros(27)
+    oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), 1.0, 0.0, 0.0, 0.0, 1.9, abi.ptr(out))
+    assert np.allclose(out, f, atol=2e-16)
+
+
+def test_collision_with_omega_one_relaxes_stress_to_equilibrium():
+    """omega = 1 relaxes every relaxable moment to equilibrium: the deviatoric
+    second central moments vanish after collision (both operators)."""
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(4)
+    f = feq(1.0, (0.02, -0.01, 0.03)) * (1 + 0.05 * rng.standard_normal(27))
+    rho = f.sum()
+    u = f @ c / rho
+    for coll in (abi.COLLISION_SRT, abi.COLLISION_CUM2015):
+        out = np.zeros(27)
+        oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), rho, *u, 1.0, abi.ptr(out))
+        cc = c - u
+        k2 = np.einsum("q,qa,qb->ab", out, cc, cc)
+        dev = k2 - np.trace(k2) / 3 * np.eye(3)
+        assert np.abs(dev).max() < 1e-5, coll
+
+
+def test_central_moment_transform_matches_definition():
+    """The three-pass transform equals the brute-force definition
+    k_ijl = sum_q (cx-ux)^i (cy-uy)^j (cz-uz)^l f_q, and its inverse undoes it."""
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(5)
+    O = oracle_lib()
+    for _ in range(10):
+        f = rng.uniform(0.0, 0.3, 27)
+        u = 0.05 * rng.standard_normal(3)
+        k = np.zeros(27)
+        O.orc_cell_central_moments(abi.ptr(f), *u, abi.ptr(k))
+        cc = c - u
+        for i, j, l in itertools.product(range(3), repeat=3):
+            ref = np.sum(cc[:, 0] ** i * cc[:, 1] ** j * cc[:, 2] ** l * f)
+            assert abs(k[9 * i + 3 * j + l] - ref) < 1e-14, (i, j, l)
+        back = np.zeros(27)
+        O.orc_cell_from_central_moments(abi.ptr(k), *u, abi.ptr(back))
+        assert np.allclose(back, f, atol=1e-15)
+
+
+def test_cumulant_relaxes_shear_at_rate_omega():
+    """Off-diagonal second cumulants relax as C* = (1 - omega) C (Eq. G2015 55-57)."""
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(6)
+    f = feq(1.0, (0.01, 0.02, -0.01)) * (1 + 0.02 * rng.standard_normal(27))
+    rho = f.sum()
+    u = f @ c / rho
+    om = 1.3
+    out = np.zeros(27)
+    oracle_lib().orc_cell_collide(abi.COLLISION_CUM2015, 0, 1, abi.ptr(f), rho, *u, om, abi.ptr(out))
+    cc = c - u
+    k_in = np.einsum("q,qa,qb->ab", f, cc, cc)
+    k_out = np.einsum("q,qa,qb->ab", out, cc, cc)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(k_out[a, b] - (1 - om) * k_in[a, b]) < 1e-15
+
+
+# --- Smagorinsky ----------------------------------------------------------
+
+def test_smagorinsky_special_cases():
+    O = oracle_lib()
+    tau = 0.52
+    f = feq(1.0, (0.02, 0.01, 0.0))
+    # equilibrium: Pi = 0 -> omega = 1/tau
+    w = O.orc_cell_omega_les(0, 1, abi.ptr(f), 1.0, 0.02, 0.01, 0.0, tau, 0.173)
+    assert abs(w - 1 / tau) < 1e-14
+    # C = 0 -> omega = 1/tau for any f
+    g = f * (1 + 0.05 * np.random.default_rng(0).standard_normal(27))
+    w = O.orc_cell_omega_les(0, 1, abi.ptr(g), 1.0, 0.02, 0.01, 0.0, tau, 0.0)
+    assert abs(w - 1 / tau) < 1e-14
+
+
+def test_smagorinsky_known_stress():
+    """Perturb the four (+-1, +-1, 0) populations so that Pi = s (e_x e_y + e_y e_x);
+    |Pi|_F = sqrt(2) s and tau_LES = tau/2 + sqrt(tau^2 + 18 C |Pi| / rho)/2."""
+    c, _, _ = tables(0)
+    O = oracle_lib()
+    rho, tau, C, s = 1.0, 0.51, 0.173, 3e-4
+    f = feq(rho, (0.0, 0.0, 0.0))
+    for q in range(27):
+        if c[q, 2] == 0 and c[q, 0] != 0 and c[q, 1] != 0:
+            f[q] += 0.25 * s * c[q, 0] * c[q, 1]
+    pi = np.einsum("q,qa,qb->ab", f - feq(rho, (0, 0, 0)), c, c)
+    assert np.allclose(pi, s * np.array([[0, 1, 0], [1, 0, 0], [0, 0, 0]]), atol=1e-18)
+    w = O.orc_cell_omega_les(0, 1, abi.ptr(f), rho, 0.0, 0.0, 0.0, tau, C)
+    tau_les = tau / 2 + 0.5 * np.sqrt(tau * tau + 18 * C * np.sqrt(2) * s / rho)
+    assert abs(w - 1 / tau_les) < 1e-13
+
+
+# --- whole-step physics ---------------------------------------------------
+
+@pytest.mark.parametrize("coll", [abi.COLLISION_SRT, abi.COLLISION_CUM2015])
+def test_shear_wave_decay_gives_viscosity(coll):
+    """u_y(x) = A sin(2 pi x / L) decays as exp(-nu k^2 t) with nu = (tau - 1/2)/3."""
+    L, tau, steps = 32, 0.8, 300
+    shape = (2, 2, L)
+    x = np.arange(L)
+    uy = np.broadcast_to(1e-3 * np.sin(2 * np.pi * x / L), shape).copy()
+    d = make_desc(np.ones(shape, np.int32), [], collision=coll, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+                  compat=0, periodic=(1, 1, 1), tau=tau)
+    o = OracleLattice(d)
+    o.init_equilibrium(None, None, uy, None)
+    o.step(50)
+    a0 = np.abs(o.macroscopic()[2][0, 0]).max()
+    o.step(steps)
+    a1 = np.abs(o.macroscopic()[2][0, 0]).max()
+    k = 2 * np.pi / L
+    nu_meas = -np.log(a1 / a0) / (k * k * steps)
+    nu = (tau - 0.5) / 3
+    assert abs(nu_meas / nu - 1) < 0.01, (nu_meas, nu)
+
+
+def test_periodic_ghost_layer_equals_index_wrap():
+    """Away from walls, the reference's ghost-layer Periodic policy
+    (Periodic.h:43-138) reproduces index-wrap periodicity bit for bit: with
+    ghost rows y = 0 (partner y = n) and y = n + 1 (partner y = 1), the interior
+    of a y-periodic box equals the index-wrapped box.  (At walls the two
+    differ: a regular ghost node, flag 10, gets no bounce-back there.)"""
+    from helpers import bc_nodes
+    from oracle import mesher_oracle as mo
+
+    n = 8
+    shape_w = (6, n, 7)
+    rng = np.random.default_rng(9)
+    flags_w = np.ones(shape_w, np.int32)
+    g_g = np.ones((6, n + 2, 7), np.int32)
+    flags_g, walls_g = mo.meshing_boundary_wall(g_g)
+    sh = flags_g.shape
+    m0 = mo.cuboid_mask(sh, (-50, -0.6, -50), (50, -0.4, 50), (-50, -0.6, 50), additional=1.0)
+    m1 = mo.cuboid_mask(sh, (-50, n + 0.4, -50), (50, n + 0.6, 50), (-50, n + 0.4, 50), additional=1.0)
+    per = mo.meshing_periodic(flags_g, m0, -3, (0.0, -1.0, 0.0), n)
+    per += mo.meshing_periodic(flags_g, m1, -3, (0.0, 1.0, 0.0), 1)
+    assert len(per) == 2 * 6 * 7 and not mo.compile_wall(flags_g, walls_g)
+    bc_g = bc_nodes([dict(kind=mo.BC_PERIODIC_GHOST, **p) for p in per])
+    ux = 0.02 * rng.standard_normal(shape_w)
+    uxg = np.concatenate([ux[:, -1:, :], ux, ux[:, :1, :]], axis=1)
+    kw = dict(collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+              compat=abi.COMPAT_REFERENCE, tau=0.6)
+    ow = OracleLattice(make_desc(flags_w, [], periodic=(1, 1, 1), **kw))
+    og = OracleLattice(make_desc(flags_g, bc_g, periodic=(1, 0, 1), **kw))
+    ow.init_equilibrium(None, ux, None, None)
+    og.init_equilibrium(None, uxg, None, None)
+    ow.step(25)
+    og.step(25)
+    assert np.array_equal(ow.populations(), og.populations()[:, :, 1:-1, :])
+
+
+def test_closed_box_mass_drift_is_the_crev_quirk():
+    """In a closed box with half-way bounce-back on every wall, mass is
+    conserved exactly with the true opposite table and drifts with the
+    reference's c_rev[20] = 21 (quirk Q1)."""
+    from helpers import bc_nodes
+    from oracle import mesher_oracle as mo
+
+    g = np.zeros((10, 10, 10), np.int32)
+    g[1:-1, 1:-1, 1:-1] = 1
+    flags, walls = mo.meshing_boundary_wall(g)
+    bc = bc_nodes([dict(kind=mo.BC_WALL, x=a, y=b, z=c, regular=r) for a, b, c, r in walls])
+    u = [a * (flags > 0) for a in random_velocity_field(flags.shape, 0.02, 1)]
+    fluid = flags > 0
+
+    def mass_change(compat):
+        o = OracleLattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
+                                    arith=abi.ARITH_LITERAL, compat=compat, tau=0.524))
+        o.init_equilibrium(None, *u)
+        o.step(1)
+        m0 = o.macroscopic()[0][fluid].sum()
+        o.step(40)
+        return o.macroscopic()[0][fluid].sum() - m0
+
+    assert abs(mass_change(abi.COMPAT_REFERENCE)) > 1e-3
+    assert abs(mass_change(abi.COMPAT_F32CONST)) < 1e-12
+    assert abs(mass_change(0)) < 1e-12
+
+
+def test_error_quadratic_known_values():
+    """ErrorQuadratic with the y-z typo of ErrorQuadratic.h:43-44: for a field
+    with uz = 0 the third term uses uy - uz_prev = uy."""
+    shape = (4, 4, 4)
+    flags = np.ones(shape, np.int32)
+    d = make_desc(flags, [], collision=abi.COLLISION_SRT, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+                  compat=0, periodic=(1, 1, 1), tau=0.8)
+    o = OracleLattice(d)
+    o.init_equilibrium(None, 0.01, 0.02, 0.0)
+    e = o.error_quadratic()  # previous u = 0: err = sqrt(sum sqrt(ux^2 + 2 uy^2)) / sqrt(sum |u|)
+    n = 64
+    expect = np.sqrt(n * np.sqrt(0.01 ** 2 + 2 * 0.02 ** 2)) / np.sqrt(n * np.sqrt(0.01 ** 2 + 0.02 ** 2))
+    assert abs(e - expect) < 1e-12
+    # u_prev now equals u, yet the typo keeps the third term (uy - uz_prev)^2 = uy^2 alive
+    e2 = o.error_quadratic()
+    assert abs(e2 - np.sqrt(0.02) / np.sqrt(np.sqrt(0.01 ** 2 + 0.02 ** 2))) < 1e-12
+
+
+# --- product arithmetic on the host --------------------------------------
+
+def test_product_collision_arithmetic_is_bitwise_the_oracle():
+    """lb_amd/csrc/collide.hpp (literal variant) compiled for the host gives
+    the oracle's numbers bit for bit; the reduced variant agrees to ~1 ulp."""
+    H, O = host_collide_lib(), oracle_lib()
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        f0 = feq(1 + 0.02 * rng.standard_normal(), 0.05 * rng.standard_normal(3)) * (1 + 0.02 * rng.standard_normal(27))
+        for dbl in (1, 0):
+            f = f0 if dbl else f0.astype(np.float32).astype(np.float64)
+            m = np.zeros(4)
+            H.host_moments(dbl, abi.ptr(f), abi.ptr(m))
+            for coll in (0, 1):
+                for compat in (0, abi.COMPAT_REFERENCE):
+                    a, b, fast = np.zeros(27), np.zeros(27), np.zeros(27)
+                    O.orc_cell_collide(coll, compat, dbl, abi.ptr(f), *m, 1.9, abi.ptr(a))
+                    H.host_collide(coll, 0, dbl, compat, abi.ptr(f), *m, 1.9, abi.ptr(b))
+                    H.host_collide(coll, 1, dbl, compat, abi.ptr(f), *m, 1.9, abi.ptr(fast))
+                    assert np.array_equal(a, b)
+                    assert np.abs(fast - a).max() < (1e-15 if dbl else 3e-7)
+            wo = O.orc_cell_omega_les(abi.COMPAT_REFERENCE, dbl, abi.ptr(f), *m, 0.5015, 0.173)
+            wh = H.host_omega_les(0, dbl, abi.COMPAT_REFERENCE, abi.ptr(f), *m, 0.5015, 0.173)
+            assert wo == wh
