@@ -137,6 +137,11 @@ lbm_status lbm_get_macroscopic(lbm_ctx* ctx, double* rho, double* ux, double* uy
  * 0 on solid cells.  Synchronises. */
 lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
 
+/* rho, ux, uy, uz of one cell (as lbm_get_macroscopic) without copying the
+ * whole field: the reference's probeWrite reads four values
+ * (SolverTurbulentLES.h:150-177).  Synchronises. */
+lbm_status lbm_probe(lbm_ctx* ctx, int32_t x, int32_t y, int32_t z, double out[4]);
+
 /* Relaxation rate omega that the next step will use on every cell, computed
  * from the current state as Turbulence::omega does at the start of a step
  * (OmegaLES.h:69-131 / OmegaNo.h:34-47); 0 on solid cells.  Synchronises. */

@@ -60,6 +60,7 @@ CORE_SYMBOLS = [
     ("lbm_sync", ctypes.c_int, [_vp]),
     ("lbm_get_macroscopic", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
     ("lbm_get_populations", ctypes.c_int, [_vp, _dp]),
+    ("lbm_probe", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]),
     ("lbm_get_omega", ctypes.c_int, [_vp, _dp]),
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),

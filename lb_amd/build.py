@@ -61,7 +61,7 @@ def build_core():
     return out
 
 
-HOST_SOURCES = []
+HOST_SOURCES = ["geometry_off.cpp", "geometry_boundary.cpp", "outputers.cpp", "solver.cpp", "capi.cpp"]
 
 
 def build_host():
@@ -70,8 +70,10 @@ def build_host():
     if not srcs:
         return None
     out = os.path.join(HERE, "liblbm_host.so")
-    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
-          f"-I{os.path.join(ROOT, 'include')}", "-o", out, *srcs])
+    # links the HIP core (its C ABI); found next to this library at run time
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-Wall",
+          f"-I{os.path.join(ROOT, 'include')}", "-o", out, *srcs, f"-L{HERE}", "-llbm_core_hip",
+          "-Wl,-rpath,$ORIGIN"])
     return out
 
 
@@ -93,8 +95,8 @@ def build_tests():
 def build_all():
     build_oracle()
     build_tests()
-    build_host()
     build_core()
+    build_host()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,252 @@
+// solver.cpp -- unit conversion and the SolverTurbulentLES time loop.
+//
+// NonDimensiolnaliseFactorsVelocity follows
+// 3D/simulation/src/solvers/nonDimensionalisions/NonDimensiolnaliseFactorsVelocity.h:20-131
+// in float arithmetic, including its tau = nu/3 + 1/2.  SolverTurbulentLES
+// follows 3D/simulation/src/solvers/SolverTurbulentLES.h:106-324: the operator
+// chain of one step (omega, collision, streaming, boundaries, moments) is a
+// single lbm_step launch; the loop keeps the reference's cadences for the
+// probe, the error norm with its NaN break, and VTK output.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+
+#include "lbm_host.hpp"
+
+namespace lbmh {
+
+LBMData::~LBMData()
+{
+    if (ctx) lbm_destroy(ctx);
+}
+
+void Timer::start()
+{
+    t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    running = true;
+}
+
+void Timer::stop()
+{
+    if (!running) return;
+    total += std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+    running = false;
+}
+
+void NonDimensiolnaliseFactorsVelocity::nonDimensionalize(LBMConstants& C, bool verbose)
+{
+    if (C.U_lb > 0.1f)
+        std::cout << "\n Lattice speed should not be higher than 0.1 due to stability close to speed of sound.\n";
+    C.L_fyz = std::abs(C.BBmaxx - C.BBminx) * C.conversion_factor_fyz;
+    C.Cl = C.L_fyz / C.dimX_int;
+    const RealType U_fyz = C.u_guess_fyz;
+    C.Cu = U_fyz / C.U_lb;
+    C.Cu_inverse = 1 / C.Cu;
+    C.Ct = C.Cl / C.Cu;
+    C.Crho = C.rho_fyz;
+    C.Cm = C.Crho * C.Cl * C.Cl * C.Cl;
+    C.Cpressure = C.Cm / C.Cl / C.Ct / C.Ct;
+    C.Re = U_fyz * C.L_fyz / C.ny_fyz;
+    C.ny = C.ny_fyz * C.Ct / C.Cl / C.Cl;
+    C.tau = C.ny / 3.f + 0.5f;  // as the reference writes it (nonDimensionalize :68)
+    if (verbose) std::cout << "\n- Re is " << C.Re << "\n- Nu is " << C.ny << "\n- Tau is " << C.tau << "\n";
+    C.omega = 1.f / C.tau;
+}
+
+void NonDimensiolnaliseFactorsVelocity::nonDimensionalizeInlet(LBMData& D, const LBMConstants& C)
+{
+    for (auto& b : D.meshBoundaryInletHost) {
+        b.velocity.x = b.velocity.x * C.Cu_inverse;
+        b.velocity.y = b.velocity.y * C.Cu_inverse;
+        b.velocity.z = b.velocity.z * C.Cu_inverse;
+    }
+}
+
+void NonDimensiolnaliseFactorsVelocity::nonDimensionalizeOutlet(LBMData& D, const LBMConstants& C)
+{
+    for (auto& b : D.meshBoundaryOutletHost) b.density = b.density / C.Cm * C.Cl * C.Cl * C.Cl;
+}
+
+void SolverTurbulentLES::convertToLattice(bool verbose) { NonDimensiolnaliseFactorsVelocity::nonDimensionalize(Constants, verbose); }
+
+// initializeSimulationData (:276-324)
+void SolverTurbulentLES::initializeSimulationData(bool verbose)
+{
+    if (Constants.plot_every_it == -1 && Constants.plot_every != -1)
+        Constants.plot_every_it = (int)std::ceil(Constants.plot_every / Constants.Ct);
+    if (Constants.err_every_it == -1 && Constants.err_every != -1)
+        Constants.err_every_it = (int)std::ceil(Constants.err_every / Constants.Ct);
+    if (Constants.iterations <= 0) Constants.iterations = (int)std::ceil(Constants.time / Constants.Ct);
+    if (Constants.probe_every_it > 0) {
+        const float ex = 2 * (Constants.additional_factor / Constants.resolution_factor);
+        Constants.ProbeLocationLat[0] =
+            (int)std::round(Constants.ProbeLocation.x / (Constants.BBmaxx - Constants.BBminx + ex) * Constants.dimX_int);
+        Constants.ProbeLocationLat[1] =
+            (int)std::round(Constants.ProbeLocation.y / (Constants.BBmaxy - Constants.BBminy + ex) * Constants.dimY_int);
+        Constants.ProbeLocationLat[2] =
+            (int)std::round(Constants.ProbeLocation.z / (Constants.BBmaxz - Constants.BBminz + ex) * Constants.dimZ_int);
+    }
+    if (verbose) {
+        std::cout << "\nPlotting every " << Constants.plot_every_it << " iterations.\n";
+        std::cout << "\nCalculation will run for " << Constants.iterations << " iterations.\n";
+    }
+}
+
+// initializeSimulation (:106-146): sizes, inlet/outlet scaling, device
+// lattice, equilibrium initialisation (InitializationEquilibriumConstVector).
+int SolverTurbulentLES::initializeSimulation(bool verbose)
+{
+    Constants.Nvel = 27;
+    initializeSimulationData(verbose);
+    NonDimensiolnaliseFactorsVelocity::nonDimensionalizeInlet(Data, Constants);
+    NonDimensiolnaliseFactorsVelocity::nonDimensionalizeOutlet(Data, Constants);
+    std::vector<lbm_bc_node> bc;
+    geometryMesherBoundary mesher(Constants, Data);
+    mesher.arrayTransfer(false, bc);
+    lbm_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.nx = Constants.dimX_int;
+    d.ny = Constants.dimY_int;
+    d.nz = Constants.dimZ_int;
+    for (int a = 0; a < 3; ++a) d.periodic[a] = Constants.periodic[a];
+    d.collision = Constants.collision;
+    d.closure = Constants.closure;
+    d.storage = Constants.storage;
+    d.arith = Constants.arith;
+    d.compat = Constants.compat;
+    d.n_bc = (int32_t)bc.size();
+    d.tau = Constants.tau;
+    d.nu = Constants.ny;
+    d.c_les = Constants.CLES;
+    d.rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
+    d.flags = Data.meshFluidHost.data();
+    d.bc = bc.empty() ? nullptr : bc.data();
+    if (Data.ctx) {
+        lbm_destroy(Data.ctx);
+        Data.ctx = nullptr;
+    }
+    int st = lbm_create(&d, &Data.ctx);
+    if (st != LBM_OK) return st;
+    // rho = rho_fyz / Cm * Cl^3 and u = u_0 * Cu^-1 on non-solid cells, inlet
+    // velocity on inlet cells (:74-109)
+    const size_t N = Data.meshFluidHost.size();
+    std::vector<double> rho(N, 0.0), ux(N, 0.0), uy(N, 0.0), uz(N, 0.0);
+    const RealType rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
+    for (size_t i = 0; i < N; ++i) {
+        if (Data.meshFluidHost[i] == 0) continue;
+        rho[i] = rho0;
+        ux[i] = (RealType)(Constants.VelocityInit.x * Constants.Cu_inverse);
+        uy[i] = (RealType)(Constants.VelocityInit.y * Constants.Cu_inverse);
+        uz[i] = (RealType)(Constants.VelocityInit.z * Constants.Cu_inverse);
+    }
+    for (const auto& b : Data.meshBoundaryInletHost) {
+        const size_t i = Data.index(b.vertex.x, b.vertex.y, b.vertex.z);
+        ux[i] = b.velocity.x;
+        uy[i] = b.velocity.y;
+        uz[i] = b.velocity.z;
+    }
+    st = lbm_init_equilibrium(Data.ctx, rho.data(), ux.data(), uy.data(), uz.data());
+    u_error.assign(3 * N, 0.0);
+    if (verbose) std::cout << "Initialization of simulation done." << std::endl;
+    return st;
+}
+
+int SolverTurbulentLES::probeWrite()
+{
+    double v[4];
+    const int st = lbm_probe(Data.ctx, Constants.ProbeLocationLat[0], Constants.ProbeLocationLat[1],
+                             Constants.ProbeLocationLat[2], v);
+    if (st != LBM_OK) return st;
+    outputerVTK::probeWriteOut(Constants, (RealType)v[1], (RealType)v[2], (RealType)v[3], (RealType)v[0]);
+    return LBM_OK;
+}
+
+// ErrorQuadratic (ErrorQuadratic.h:33-90) on fluid cells with flag == 1,
+// including the third-component typo (uy - u_prev.z); summed in double.
+int SolverTurbulentLES::errorEvaluation()
+{
+    const size_t N = Data.meshFluidHost.size();
+    Data.rho_out.resize(N);
+    Data.ux_out.resize(N);
+    Data.uy_out.resize(N);
+    Data.uz_out.resize(N);
+    const int st = lbm_get_macroscopic(Data.ctx, Data.rho_out.data(), Data.ux_out.data(), Data.uy_out.data(),
+                                       Data.uz_out.data());
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    double e1 = 0, e2 = 0;
+    for (size_t i = 0; i < N; ++i) {
+        if (Data.meshFluidHost[i] != 1) continue;
+        const double ux = Data.ux_out[i], uy = Data.uy_out[i], uz = Data.uz_out[i];
+        const double dx = ux - u_error[3 * i], dy = uy - u_error[3 * i + 1], dz = uy - u_error[3 * i + 2];
+        e1 += std::sqrt(dx * dx + dy * dy + dz * dz);
+        e2 += std::sqrt(ux * ux + uy * uy + uz * uz);
+    }
+    Constants.err = (RealType)(std::sqrt(e1) / std::sqrt(e2));
+    for (size_t i = 0; i < N; ++i) {
+        u_error[3 * i] = Data.ux_out[i];
+        u_error[3 * i + 1] = Data.uy_out[i];
+        u_error[3 * i + 2] = Data.uz_out[i];
+    }
+    err_history.push_back(Constants.err);
+    return st;
+}
+
+// runSimulation (:179-268).  Steps between two host events are queued as one
+// lbm_step call; the event cadences are those of the reference loop.
+int SolverTurbulentLES::runSimulation()
+{
+    if (!Data.ctx) return LBM_E_STATE;
+    timer_loop.start();
+    int k = 0;
+    const int iters = Constants.iterations;
+    auto is_event = [&](int kk) {
+        const bool probe = Constants.probe_every_it > 0 && kk % Constants.probe_every_it == 0 &&
+                           kk >= (iters - Constants.probe_iterations);
+        const bool err = Constants.err_every_it > 0 && kk % Constants.err_every_it == 0 && kk != 0;
+        const bool plot = write_output && Constants.plot_every_it > 0 && kk % Constants.plot_every_it == 0 && kk;
+        return probe || err || plot;
+    };
+    int status = LBM_OK;
+    while (k < iters) {
+        // advance to the end of step k, or further while no host event is due
+        int kk = k;
+        while (kk + 1 < iters && !is_event(kk)) ++kk;
+        timer_collision.start();
+        status = lbm_step(Data.ctx, kk - k + 1);
+        if (status == LBM_OK) status = lbm_sync(Data.ctx);
+        timer_collision.stop();
+        if (status != LBM_OK) break;
+        k = kk;
+        if (Constants.probe_every_it > 0 && k % Constants.probe_every_it == 0 &&
+            k >= (iters - Constants.probe_iterations)) {
+            timer_output.start();
+            status = probeWrite();
+            timer_output.stop();
+            if (status != LBM_OK) break;
+        }
+        if (Constants.err_every_it > 0 && k % Constants.err_every_it == 0 && k != 0) {
+            timer_err.start();
+            status = errorEvaluation();
+            timer_err.stop();
+            std::printf("\n err=%e k=%d\n", (double)Constants.err, k);
+            if (std::isnan(Constants.err)) {
+                std::cout << "\n Error is NaN, breaking out.\n";
+                status = LBM_E_NAN;
+                break;
+            }
+        }
+        if (write_output && Constants.plot_every_it > 0 && k % Constants.plot_every_it == 0 && k) {
+            timer_output.start();
+            outputerVTK::variablesVTK(Data, Constants, k / Constants.plot_every_it, true);
+            timer_output.stop();
+        }
+        k++;
+    }
+    timer_loop.stop();
+    return status;
+}
+
+}  // namespace lbmh

@@ -525,6 +525,29 @@ __global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
     a.io[4 * N + i] = (double)om;
 }
 
+// rho, u of a single cell (probe).
+template <class ST>
+__global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
+{
+    using R = typename ST::R;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t i = (uint32_t)(x + a.g.nx * (y + a.g.ny * z));
+    const uint8_t cls = a.cls[i];
+    if (cls == CL_SOLID) {
+        for (int k = 0; k < 4; ++k) a.io[k] = 0.0;
+        return;
+    }
+    const Cell c = aux_neigh(a.g, x, y, z, i);
+    R f[Q];
+    cell_state(a, c, cls, f);
+    R rho, ux, uy, uz;
+    moments_literal(f, rho, ux, uy, uz);
+    a.io[0] = (double)rho;
+    a.io[1] = (double)ux;
+    a.io[2] = (double)uy;
+    a.io[3] = (double)uz;
+}
+
 // Per-block FP64 sums of mass and momentum over non-solid cells.
 template <class ST>
 __global__ __launch_bounds__(TPB) void k_sums(AuxArgs<ST> a)
@@ -1101,6 +1124,43 @@ lbm_status lbm_get_macroscopic(lbm_ctx* c, double* rho, double* ux, double* uy, 
     for (int k = 0; k < 4; ++k)
         if (outs[k])
             HIPCHK(hipMemcpyAsync(outs[k], c->stage + k * N, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_probe(lbm_ctx* c, int32_t x, int32_t y, int32_t z, double out[4])
+{
+    if (!c || !out) return LBM_E_ARG;
+    if (x < 0 || y < 0 || z < 0 || x >= c->g.nx || y >= c->g.ny || z >= c->g.nz) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    lbm_status st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    if (c->first) {
+        double tmp[4];
+        const size_t i = (size_t)x + (size_t)c->g.nx * ((size_t)y + (size_t)c->g.ny * (size_t)z);
+        uint8_t cls = 0;
+        HIPCHK(hipMemcpy(&cls, c->cls + i, 1, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 4; ++k) {
+            if (c->rbytes == 8) {
+                HIPCHK(hipMemcpy(&tmp[k], (double*)c->macro + k * c->N + i, 8, hipMemcpyDeviceToHost));
+            } else {
+                float v;
+                HIPCHK(hipMemcpy(&v, (float*)c->macro + k * c->N + i, 4, hipMemcpyDeviceToHost));
+                tmp[k] = v;
+            }
+            out[k] = cls == CL_SOLID ? 0.0 : tmp[k];
+        }
+        return LBM_OK;
+    }
+    st = with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        k_probe<ST><<<1, 64, 0, c->stream>>>(a, x, y, z);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (st != LBM_OK) return st;
+    HIPCHK(hipMemcpyAsync(out, c->stage, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }

@@ -1,0 +1,215 @@
+"""Host-side setup library (no GPU): voxelizer, flag-grid files, boundary
+mesher, unit conversion.  Checked against the reference's committed
+voxelization, the oracle mesher / brute-force voxelizer, and analytic shapes."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from helpers import ROOT
+from lb_amd import abi, host
+from lb_amd.geometry import edge_manifold, mesh_volume, obstacle_scene, uv_ellipsoid, write_off
+from oracle import mesher_oracle as mo
+from test_voxelizer import GOLD_OFF, GOLD_TXT, OUTSIDE, golden_grid, oracle_voxelize
+
+
+def test_voxelizer_matches_reference_grid_bit_exact():
+    dims, gold = golden_grid()
+    g, mn, mx = host.voxelize_off(GOLD_OFF, 0.25, 1.0, OUTSIDE)
+    assert list(g.shape[::-1]) == dims
+    assert mn == (0.0, 0.0, 0.0) and mx == (1500.0, 200.0, 200.0)
+    assert np.array_equal(g, gold)
+
+
+def test_mesh_file_is_byte_identical_to_reference(tmp_path):
+    g, mn, mx = host.voxelize_off(GOLD_OFF, 0.25, 1.0, OUTSIDE)
+    p = str(tmp_path / "BackwardStepLaminar.txt")
+    host.mesh_write(p, g, 0.25, 1.0, mn, mx)
+    assert open(p, "rb").read() == gzip.open(GOLD_TXT, "rb").read()
+    g2, res, add, mn2, mx2 = host.mesh_read(p)
+    assert np.array_equal(g2, g) and res == 0.25 and add == 1.0 and mn2 == mn and mx2 == mx
+
+
+def test_mesh_read_rejects_truncated_and_missing(tmp_path):
+    p = tmp_path / "t.txt"
+    data = gzip.open(GOLD_TXT, "rb").read()
+    p.write_bytes(data[: len(data) // 2])
+    with pytest.raises(RuntimeError):
+        host.mesh_read(str(p))
+    with pytest.raises(RuntimeError):
+        host.mesh_read(str(tmp_path / "missing.txt"))
+
+
+def test_voxelizer_rejects_empty_and_malformed(tmp_path):
+    p = tmp_path / "e.off"
+    p.write_text("OFF\n0 0 0\n")
+    with pytest.raises(RuntimeError, match="empty"):
+        host.voxelize_off(str(p), 1.0, 1.0, OUTSIDE)
+    p.write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n")
+    with pytest.raises(RuntimeError, match="unreadable"):
+        host.voxelize_off(str(p), 1.0, 1.0, OUTSIDE)
+
+
+def test_synthetic_obstacle_scene_properties():
+    v, f, objs = obstacle_scene()
+    assert 19000 <= len(f) <= 21000
+    assert edge_manifold(f)
+    sv, sf = uv_ellipsoid((0, 0, 0), (24, 24, 24), 42, 60)
+    # an inscribed polyhedron: volume slightly below the sphere's, within 1 %
+    assert 0.99 < mesh_volume(sv, sf) / (4 / 3 * np.pi * 24 ** 3) < 1.0
+    # obstacles inside the box, separated
+    for k, (_, c, r) in enumerate(objs):
+        assert np.all(c - r.max() > 0) and np.all(c + r.max() < np.array([512, 256, 256]))
+        for _, c2, r2 in objs[k + 1:]:
+            assert np.linalg.norm(c - c2) > r.max() + r2.max()
+
+
+def test_voxelizer_against_oracle_and_analytic_shapes(tmp_path):
+    """A small scene (box + sphere + ellipsoid): product == brute-force oracle
+    everywhere, and == the analytic inside test away from the surfaces."""
+    from lb_amd.geometry import box, merge
+
+    parts = [box((0.0, 0.0, 0.0), (48.0, 32.0, 32.0)), uv_ellipsoid((14.0, 15.0, 17.0), (7.0, 7.0, 7.0), 24, 32),
+             uv_ellipsoid((33.0, 16.0, 14.0), (6.0, 9.0, 5.0), 24, 32)]
+    v, f = merge(parts)
+    p = str(tmp_path / "scene.off")
+    write_off(p, v, f)
+    out = (-3.1416, -2.7183, -1.4142)
+    g, mn, mx = host.voxelize_off(p, 1.0, 1.0, out)
+    dims, go = oracle_voxelize(p, 1.0, 1.0, out)
+    assert np.array_equal(g, go)
+    nz, ny, nx = g.shape
+    z, y, x = np.meshgrid(np.arange(nz) - 0.5, np.arange(ny) - 0.5, np.arange(nx) - 0.5, indexing="ij")
+    s1 = ((x - 14) ** 2 + (y - 15) ** 2 + (z - 17) ** 2) / 49.0
+    s2 = ((x - 33) / 6.0) ** 2 + ((y - 16) / 9.0) ** 2 + ((z - 14) / 5.0) ** 2
+    inbox = (x > 0) & (x < 48) & (y > 0) & (y < 32) & (z > 0) & (z < 32)
+    analytic = inbox & (s1 > 1) & (s2 > 1)
+    far = (np.abs(np.sqrt(s1) - 1) > 0.15) & (np.abs(np.sqrt(s2) - 1) > 0.25) & \
+          (np.minimum.reduce([np.abs(x), np.abs(x - 48), np.abs(y), np.abs(y - 32), np.abs(z), np.abs(z - 32)]) > 0.25)
+    assert np.array_equal(g[far].astype(bool), analytic[far])
+    assert far.mean() > 0.8
+
+
+def _mesh_compare(grid, setup):
+    """Run the same boundary meshing on the product mesher and the oracle
+    mesher; compare flags and boundary lists."""
+    m = host.Mesher(grid, 1.0, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    of, walls = mo.meshing_boundary_wall(grid)
+    obc = []
+    for kind, args in setup:
+        if kind == "inlet":
+            p1, p2, p3, cid, nrm, vel = args
+            assert m.inlet(p1, p2, p3, cid, nrm, vel, abi.BC_INLET_MOVING_WALL) == 0
+            mask = mo.cuboid_mask(grid.shape, p1, p2, p3, lookup_dtype=np.float32)
+            obc += [dict(kind=abi.BC_INLET_MOVING_WALL, **d) for d in mo.meshing_inlet_uniform(of, mask, cid, nrm, vel)]
+        elif kind == "outlet":
+            p1, p2, p3, cid, nrm, dens = args
+            m.outlet(p1, p2, p3, cid, nrm, dens)
+            mask = mo.cuboid_mask(grid.shape, p1, p2, p3)
+            obc += [dict(kind=abi.BC_OUTLET_INTERPOLATED, **d) for d in mo.meshing_outlet(of, mask, cid, nrm, dens)]
+        elif kind == "periodic":
+            p1, p2, p3, cid, nrm, pi = args
+            m.periodic(p1, p2, p3, cid, nrm, pi)
+            mask = mo.cuboid_mask(grid.shape, p1, p2, p3)
+            obc += [dict(kind=abi.BC_PERIODIC_GHOST, **d) for d in mo.meshing_periodic(of, mask, cid, nrm, pi)]
+    walls = mo.compile_wall(of, walls)
+    flags, bc = m.compile()
+    assert np.array_equal(flags, of)
+    pw = [(b.x, b.y, b.z, b.regular) for b in bc if b.kind == abi.BC_WALL]
+    assert pw == [tuple(w) for w in walls]
+    # the product lists inlets, then periodic, then outlets (compile order)
+    order = {abi.BC_INLET_MOVING_WALL: 0, abi.BC_PERIODIC_GHOST: 1, abi.BC_OUTLET_INTERPOLATED: 2}
+    pn = [(b.kind, b.x, b.y, b.z, b.regular) for b in bc if b.kind != abi.BC_WALL]
+    on = sorted([(d["kind"], d["x"], d["y"], d["z"], d["regular"]) for d in obc],
+                key=lambda t: order[t[0]])
+    assert pn == on
+    return flags, bc
+
+
+def test_mesher_matches_oracle_cavity():
+    n = 10
+    g = np.zeros((n + 2, n + 2, n + 2), np.int32)
+    g[1:-1, 1:-1, 1:-1] = 1
+    flags, bc = _mesh_compare(g, [("inlet", ((-1.0, n - 0.75, -1.0), (n + 1.0, n - 0.25, n + 1.0),
+                                              (-1.0, n - 0.75, n + 1.0), -1, (0.0, 1.0, 0.0), (0.05, 0.0, 0.0)))])
+    # closed forms: face cells 1 + 9, edges 1 + 15, corners 1 + 19
+    assert flags[1, 5, 5] == 10 and flags[1, 1, 5] == 16 and flags[1, 1, 1] == 20 and flags[5, 5, 5] == 1
+    assert np.all(flags[1:-1, n, 1:-1] == -1)
+
+
+def test_mesher_matches_oracle_voxelized_step():
+    """Boundary meshing of the reference's own backward-step grid with an
+    inlet, an outlet and two periodic cuboids."""
+    dims, gold = golden_grid()
+    setup = [
+        # cell centres are at 4 i - 2 (resolution 0.25, one halo layer): the first
+        # fluid layer sits at 2, the last one along x at 1498
+        ("periodic", ((-1.0, 1.0, -1.0), (1600.0, 3.0, 300.0), (-1.0, 1.0, 300.0), -3, (0.0, -1.0, 0.0), 50)),
+        ("inlet", ((1.0, -5.0, -5.0), (3.0, 250.0, 250.0), (1.0, 250.0, -5.0), -1, (-1.0, 0.0, 0.0),
+                   (0.04, 0.0, 0.0))),
+        ("outlet", ((1497.0, -5.0, -5.0), (1499.0, 250.0, 250.0), (1497.0, 250.0, -5.0), -2, (1.0, 0.0, 0.0), 1.0)),
+    ]
+    # the reference mesher works in physical coordinates: resolution 0.25, additional 1
+    m = host.Mesher(gold, 0.25, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    of, walls = mo.meshing_boundary_wall(gold)
+    for kind, args in setup:
+        p1, p2, p3, cid, nrm, last = args
+        if kind == "inlet":
+            assert m.inlet(p1, p2, p3, cid, nrm, last) == 0
+            mo.meshing_inlet_uniform(of, mo.cuboid_mask(gold.shape, p1, p2, p3, 0.25, 1.0, lookup_dtype=np.float32),
+                                     cid, nrm, last)
+        elif kind == "outlet":
+            m.outlet(p1, p2, p3, cid, nrm, last)
+            mo.meshing_outlet(of, mo.cuboid_mask(gold.shape, p1, p2, p3, 0.25, 1.0), cid, nrm, last)
+        else:
+            m.periodic(p1, p2, p3, cid, nrm, last)
+            mo.meshing_periodic(of, mo.cuboid_mask(gold.shape, p1, p2, p3, 0.25, 1.0), cid, nrm, last)
+    flags, bc = m.compile()
+    assert np.array_equal(flags, of)
+    assert (flags == -1).sum() > 0 and (flags == -2).sum() > 0 and (flags == -3).sum() > 0
+    assert sum(b.kind == abi.BC_WALL for b in bc) == len(mo.compile_wall(of, walls))
+
+
+def test_inlet_with_equal_velocity_components_is_rejected():
+    g = np.zeros((6, 6, 6), np.int32)
+    g[1:-1, 1:-1, 1:-1] = 1
+    m = host.Mesher(g)
+    m.wall()
+    assert m.inlet((0, 0, 0), (9, 9, 9), (0, 9, 9), -1, (1, 0, 0), (0.0, 0.0, 0.0)) == 1
+
+
+def test_nondimensionalize_reference_arithmetic():
+    """Unit conversion of NonDimensiolnaliseFactorsVelocity in float, with its
+    tau = nu_lb / 3 + 1/2."""
+    L = host.load_host()
+    import ctypes
+
+    h = ctypes.c_void_p()
+    assert L.lbmh_case_create(ctypes.byref(h)) == 0
+    g = np.ones((4, 4, 100), np.int32)
+    dims = (ctypes.c_int32 * 3)(100, 4, 4)
+    assert L.lbmh_case_set_grid(h, dims, g.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 100.0, 0.0,
+                                (ctypes.c_float * 3)(0, 0, 0), (ctypes.c_float * 3)(1, 0.04, 0.04)) == 0
+    for k, v in (("u_guess_fyz", 1.0), ("U_lb", 0.1), ("ny_fyz", 1e-3), ("rho_fyz", 1000.0)):
+        assert L.lbmh_case_set(h, k.encode(), v) == 0
+    assert L.lbmh_case_convert_to_lattice(h) == 0
+
+    def get(k):
+        v = ctypes.c_double()
+        assert L.lbmh_case_get(h, k.encode(), ctypes.byref(v)) == 0
+        return v.value
+
+    f = np.float32
+    Cl = f(1.0) / f(100)
+    Cu = f(1.0) / f(0.1)
+    Ct = Cl / Cu
+    ny = f(1e-3) * Ct / Cl / Cl
+    assert get("Cl") == Cl and get("Cu") == Cu and get("Ct") == Ct and get("ny") == ny
+    assert get("tau") == ny / f(3) + f(0.5)
+    assert get("omega") == f(1) / (ny / f(3) + f(0.5))
+    assert abs(get("Re") - 1000.0) < 1e-3
+    assert L.lbmh_case_set(h, b"no_such_field", 1.0) == 1
+    L.lbmh_case_destroy(h)
