@@ -1,606 +1,23 @@
-// lbm_core.hip -- MI355X (gfx950) lattice-Boltzmann collide-stream core.
+// lbm_core.hip -- host side of the MI355X (gfx950) collide-stream core:
+// flag-grid classification, device buffers, launches and the C ABI of
+// include/lbm_core.h.  The device code is in lbm_kernels.hpp.
 //
-// One fused kernel per time step: load the 27 populations of a cell, derive
-// rho/u, the relaxation rate (constant or Smagorinsky), collide (cumulant
-// 2015 or BGK) and stream in place with the A-A ("esoteric twist") pattern,
-// applying bounce-back and moving-wall boundaries from the flag grid in the
-// same pass.  It replaces the reference's per-step operator chain
-// OmegaLES -> Collision -> StreamingAB -> BounceBackWallHalf -> Inlet ->
-// MomentDensityVelocityN27 (SolverTurbulentLES.h:187-214), which reads and
-// writes two full lattices plus rho/u/omega every step.
-//
-// A-A layout: a single population array P[q][cell] (structure of arrays,
-// x fastest).  With t the step counter:
-//   even t: cell x reads f_q from P[q][x] and writes post-collision f*_q to
-//           P[opp q][x];
-//   odd t:  cell x reads f_q from P[opp q][x - c_q] and writes f*_q to
-//           P[q][x + c_q].
-// Every slot is read and written by the same thread within a step, so the
-// scheme is race-free without a second lattice.
-//
-// Boundaries (reference semantics, see the oracle for the AB restatement):
-//  * source x - c_q solid (flag 0): the cell itself writes the slot its
-//    neighbour would have written -- f*_{crev q} for wall-list nodes
-//    (half-way bounce-back, BounceBackWallHalf.h:46-47) or its own
-//    pre-collision f_q otherwise (no bounce-back: StreamingAB.h:58-65 leaves
-//    the slot untouched);
-//  * destination x + c_q solid or inlet: the streaming write is skipped;
-//  * inlet cells (flag -1) keep their own state in a side buffer, stream
-//    their pre-collision populations (StreamingAB.h:62-65) and update it with
-//    the moving-wall rule (InletVelocityMovingWall.h:43-49).
+// Lattice in HBM: the caller's nx*ny*nz cells plus a one-cell solid halo on
+// every axis without index wrap, so that populations streaming to or from
+// outside the caller's lattice have a slot to live in (the reference reads
+// and writes outside its arrays there, StreamingAB.h:51-56; the oracle
+// leaves such populations untouched, which the halo reproduces).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <algorithm>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
-#include "../../include/lbm_core.h"
-#include "collide.hpp"
-#include "storage.hpp"
-
-namespace lbm {
-
-enum : uint8_t { CL_SOLID = 0, CL_FLUID = 1, CL_BND = 2, CL_INLET_MW = 3 };
-
-constexpr int TPB = 256;
-constexpr uint64_t MASK27 = (1ull << 27) - 1;
-
-struct Geo {
-    int nx, ny, nz;
-    int bx_log2, by;   // tile: (1 << bx_log2) x by cells, 256 threads
-    int ntx, nty;      // tiles along x and y
-    int ntiles;
-    int wx, wy, wz;    // index wrap per axis
-};
-
-struct InletDev {
-    float nx, ny, nz;
-    float ux, uy, uz;
-    int32_t kind;
-    int32_t pad;
-};
-
-template <class ST>
-struct StepArgs {
-    typename ST::T* p[Q];
-    const uint8_t* cls;
-    const uint64_t* node;
-    const int32_t* bcidx;
-    typename ST::R* inl_state;      // [n_inlet][27]
-    const InletDev* inl;
-    const typename ST::R* macro;    // rho, ux, uy, uz planes for the first step
-    double* partials;               // [ntiles][4] when diag != 0
-    Geo g;
-    Consts<typename ST::R> K;
-    typename ST::R rest[4];         // w*rho0 per weight class (deviation storage)
-    typename ST::R omega0, tau, cles;
-    int crev20;
-    int first;
-    int diag;
-    int closure;
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs; renumber so that
-// consecutive tiles (neighbouring x segments, which share partial cache
-// lines on odd steps) land on the same XCD and its L2.
-__device__ __forceinline__ int xcd_tile(int bid, int G)
-{
-    const int k = G >> 3, r = G & 7;
-    const int x = bid & 7, y = bid >> 3;
-    return (x < r) ? x * (k + 1) + y : r * (k + 1) + (x - r) * k + y;
-}
-
-struct Cell {
-    int x, y, z;
-    uint32_t i;
-    int dxm, dxp, dym, dyp, dzm, dzp;
-    bool inside;
-};
-
-__device__ __forceinline__ Cell cell_of(const Geo& g)
-{
-    Cell c;
-    const int t = xcd_tile((int)blockIdx.x, g.ntiles);
-    const int tx = t % g.ntx;
-    const int rem = t / g.ntx;
-    const int ty = rem % g.nty;
-    c.z = rem / g.nty;
-    c.x = (tx << g.bx_log2) + ((int)threadIdx.x & ((1 << g.bx_log2) - 1));
-    c.y = ty * g.by + ((int)threadIdx.x >> g.bx_log2);
-    c.inside = c.x < g.nx && c.y < g.ny;
-    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
-    c.i = (uint32_t)(c.x + g.nx * (c.y + g.ny * c.z));
-    c.dxm = (c.x > 0 || !g.wx) ? -sx : (g.nx - 1) * sx;
-    c.dxp = (c.x < g.nx - 1 || !g.wx) ? sx : -(g.nx - 1) * sx;
-    c.dym = (c.y > 0 || !g.wy) ? -sy : (g.ny - 1) * sy;
-    c.dyp = (c.y < g.ny - 1 || !g.wy) ? sy : -(g.ny - 1) * sy;
-    c.dzm = (c.z > 0 || !g.wz) ? -sz : (g.nz - 1) * sz;
-    c.dzp = (c.z < g.nz - 1 || !g.wz) ? sz : -(g.nz - 1) * sz;
-    return c;
-}
-
-// Index of x + c_q.
-__device__ __forceinline__ uint32_t nb(const Cell& c, int q)
-{
-    int d = 0;
-    d += CX[q] > 0 ? c.dxp : (CX[q] < 0 ? c.dxm : 0);
-    d += CY[q] > 0 ? c.dyp : (CY[q] < 0 ? c.dym : 0);
-    d += CZ[q] > 0 ? c.dzp : (CZ[q] < 0 ? c.dzm : 0);
-    return (uint32_t)((int)c.i + d);
-}
-
-// Read the pre-collision populations of a streamed cell at step parity ODD.
-template <class ST, bool ODD>
-__device__ __forceinline__ void load_pop(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
-                                         typename ST::R* f)
-{
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (ODD)
-            f[q] = ST::load(p[OPP[q]], nb(c, OPP[q]), rest[WCLASS[q]]);
-        else
-            f[q] = ST::load(p[q], c.i, rest[WCLASS[q]]);
-    }
-}
-
-// Streaming write of post-collision population q (goes to x + c_q).
-template <class ST, bool ODD>
-__device__ __forceinline__ void store_out(typename ST::T* const* p, const typename ST::R* rest, const Cell& c, int q,
-                                          typename ST::R v)
-{
-    if (ODD)
-        ST::store(p[q], nb(c, q), v, rest[WCLASS[q]]);
-    else
-        ST::store(p[OPP[q]], c.i, v, rest[WCLASS[q]]);
-}
-
-// Write the value cell x will read as incoming population q next step
-// (used when x - c_q is solid and nobody else writes that slot).
-template <class ST, bool ODD>
-__device__ __forceinline__ void store_in_next(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
-                                              int q, typename ST::R v)
-{
-    if (ODD)
-        ST::store(p[q], c.i, v, rest[WCLASS[q]]);
-    else
-        ST::store(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]]);
-}
-
-// Moving-wall update of an inlet cell's own state
-// (InletVelocityMovingWall.h:43-49): for n.c_q > 0,
-// f(x, c_rev[q]) = f*(x, q) - 6 w_q rho (c_q . u_w).  The dot products are
-// float as in the reference (its normals and velocities are float vectors).
-template <class R>
-__device__ __forceinline__ void moving_wall(const Consts<R>& K, const InletDev& b, bool crev20, const R* post, R rho,
-                                            R* state)
-{
-    LBM_FP_STRICT
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const float nc = b.nx * (float)CX[q] + b.ny * (float)CY[q] + b.nz * (float)CZ[q];
-        if (nc > 0.0f) {
-            const float cu = (float)CX[q] * b.ux + (float)CY[q] * b.uy + (float)CZ[q] * b.uz;
-            const R v = post[q] - R(6) * K.w[WCLASS[q]] * rho * (R)cu;
-            if (q == 20) {
-                if (crev20) state[21] = v; else state[19] = v;
-            } else {
-                state[OPP[q]] = v;
-            }
-        }
-    }
-}
-
-template <class R>
-__device__ __forceinline__ R fast_rcp(R x)
-{
-    return R(1) / x;
-}
-
-template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS>
-__global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
-{
-    using R = typename ST::R;
-    const Cell c = cell_of(a.g);
-    bool active = c.inside;
-    uint8_t cls = CL_FLUID;
-    if (FLAGS && active) cls = a.cls[c.i];
-    if (cls == CL_SOLID) active = false;
-    double dm[4] = {0.0, 0.0, 0.0, 0.0};
-
-    if (active) {
-        R f[Q];
-        int bi = -1;
-        if (FLAGS && cls == CL_INLET_MW) {
-            bi = a.bcidx[c.i];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
-        } else {
-            load_pop<ST, ODD>(a.p, a.rest, c, f);
-        }
-
-        R rho, ux, uy, uz, omega;
-        R P[Q];
-        if (LIT) {
-            if (a.first) {
-                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-                rho = a.macro[c.i];
-                ux = a.macro[N + c.i];
-                uy = a.macro[2 * N + c.i];
-                uz = a.macro[3 * N + c.i];
-            } else {
-                moments_literal(f, rho, ux, uy, uz);
-            }
-            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
-                                                         : a.omega0;
-#pragma unroll
-            for (int q = 0; q < Q; ++q) P[q] = f[q];
-            if (COLL == LBM_COLLISION_CUM2015)
-                cum2015_literal(a.K, P, rho, ux, uy, uz, omega);
-            else
-                srt_literal(a.K, P, rho, ux, uy, uz, omega);
-            if (a.diag) {
-                dm[0] = (double)rho;
-                dm[1] = (double)(rho * ux);
-                dm[2] = (double)(rho * uy);
-                dm[3] = (double)(rho * uz);
-            }
-        } else {
-            const Raw2<R> r = raw_moments(f);
-            if (a.first) {
-                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-                rho = a.macro[c.i];
-                ux = a.macro[N + c.i];
-                uy = a.macro[2 * N + c.i];
-                uz = a.macro[3 * N + c.i];
-            } else {
-                rho = r.m0;
-                const R ir = fast_rcp(rho);
-                ux = r.jx * ir;
-                uy = r.jy * ir;
-                uz = r.jz * ir;
-            }
-            const Cen2<R> ce = central2(r, ux, uy, uz);
-            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
-            if (COLL == LBM_COLLISION_CUM2015) {
-                cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; ++q) P[q] = f[q];
-                srt_fast(a.K, P, rho, ux, uy, uz, omega);
-            }
-            if (a.diag) {
-                dm[0] = (double)r.m0;
-                dm[1] = (double)r.jx;
-                dm[2] = (double)r.jy;
-                dm[3] = (double)r.jz;
-            }
-        }
-
-        if (!FLAGS || cls == CL_FLUID) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
-        } else {
-            const uint64_t nw = a.node[c.i];
-            const uint32_t S = (uint32_t)(nw & MASK27);
-            const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
-            const bool wall = (nw >> 63) != 0;
-            if (cls == CL_INLET_MW) {
-                // stream the pre-collision state, keep the updated one
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
-                R st[Q];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) st[q] = f[q];
-                moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    if ((S >> q) & 1u) {
-                        R v;
-                        if (!wall)
-                            v = f[q];
-                        else if (q == 20)
-                            v = a.crev20 ? P[21] : P[19];
-                        else
-                            v = P[OPP[q]];
-                        store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
-                    }
-                }
-            }
-        }
-    }
-
-    if (a.diag) {
-        // wave-level FP64 reduction, then across the 4 waves of the block
-        __shared__ double red[TPB / 64][4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            double v = dm[k];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            dm[k] = v;
-        }
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) red[wv][k] = dm[k];
-        }
-        __syncthreads();
-        if (threadIdx.x < 4) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
-            a.partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Auxiliary kernels: initialisation, gather/scatter, moments, omega, sums.
-// ---------------------------------------------------------------------------
-
-template <class ST>
-struct AuxArgs {
-    typename ST::T* p[Q];
-    const uint8_t* cls;
-    const int32_t* bcidx;
-    typename ST::R* inl_state;
-    typename ST::R* macro;   // rho, ux, uy, uz planes
-    Geo g;
-    Consts<typename ST::R> K;
-    typename ST::R rest[4];
-    typename ST::R tau, cles, omega0;
-    int closure;
-    int lit;
-    int odd;        // parity of the stored state
-    int q;          // plane for gather/scatter
-    double* io;     // staging buffer (FP64)
-};
-
-__device__ __forceinline__ uint32_t aux_cell(const Geo& g, int& x, int& y, int& z, bool& ok)
-{
-    const size_t N = (size_t)g.nx * g.ny * g.nz;
-    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
-    ok = i < N;
-    const uint32_t ii = (uint32_t)(ok ? i : 0);
-    x = (int)(ii % (uint32_t)g.nx);
-    const uint32_t r = ii / (uint32_t)g.nx;
-    y = (int)(r % (uint32_t)g.ny);
-    z = (int)(r / (uint32_t)g.ny);
-    return ii;
-}
-
-__device__ __forceinline__ Cell aux_neigh(const Geo& g, int x, int y, int z, uint32_t i)
-{
-    Cell c;
-    c.x = x; c.y = y; c.z = z; c.i = i; c.inside = true;
-    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
-    c.dxm = (x > 0 || !g.wx) ? -sx : (g.nx - 1) * sx;
-    c.dxp = (x < g.nx - 1 || !g.wx) ? sx : -(g.nx - 1) * sx;
-    c.dym = (y > 0 || !g.wy) ? -sy : (g.ny - 1) * sy;
-    c.dyp = (y < g.ny - 1 || !g.wy) ? sy : -(g.ny - 1) * sy;
-    c.dzm = (z > 0 || !g.wz) ? -sz : (g.nz - 1) * sz;
-    c.dzp = (z < g.nz - 1 || !g.wz) ? sz : -(g.nz - 1) * sz;
-    return c;
-}
-
-// Current populations of a non-solid cell.
-template <class ST>
-__device__ __forceinline__ void cell_state(const AuxArgs<ST>& a, const Cell& c, uint8_t cls,
-                                           typename ST::R* f)
-{
-    if (cls == CL_INLET_MW) {
-        const int bi = a.bcidx[c.i];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
-    } else if (a.odd) {
-        load_pop<ST, true>(a.p, a.rest, c, f);
-    } else {
-        load_pop<ST, false>(a.p, a.rest, c, f);
-    }
-}
-
-// f = f_eq(rho, u) from the macro planes (InitializationEquilibriumConstVector.h:111-126).
-template <class ST>
-__global__ __launch_bounds__(TPB) void k_init(AuxArgs<ST> a)
-{
-    using R = typename ST::R;
-    int x, y, z;
-    bool ok;
-    const uint32_t i = aux_cell(a.g, x, y, z, ok);
-    if (!ok) return;
-    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    const uint8_t cls = a.cls[i];
-    const R rho = a.macro[i], ux = a.macro[N + i], uy = a.macro[2 * N + i], uz = a.macro[3 * N + i];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const R v = cls == CL_SOLID ? R(0) : feq_q(a.K, q, ux, uy, uz, rho);
-        ST::store(a.p[q], i, v, a.rest[WCLASS[q]]);
-        if (cls == CL_INLET_MW) a.inl_state[(size_t)a.bcidx[i] * Q + q] = v;
-    }
-}
-
-// Write plane q of a host-provided state (FP64 staging) at the current parity.
-template <class ST>
-__global__ __launch_bounds__(TPB) void k_scatter(AuxArgs<ST> a)
-{
-    using R = typename ST::R;
-    int x, y, z;
-    bool ok;
-    const uint32_t i = aux_cell(a.g, x, y, z, ok);
-    if (!ok) return;
-    const uint8_t cls = a.cls[i];
-    if (cls == CL_SOLID) return;
-    const int q = a.q;
-    const R v = (R)a.io[i];
-    if (cls == CL_INLET_MW) {
-        a.inl_state[(size_t)a.bcidx[i] * Q + q] = v;
-        return;
-    }
-    const Cell c = aux_neigh(a.g, x, y, z, i);
-    // same slot as load_pop reads
-    uint32_t j = i;
-    int plane = q;
-    if (a.odd) {
-        plane = OPP[q];
-        int d = 0;
-        const int o = OPP[q];
-        d += CX[o] > 0 ? c.dxp : (CX[o] < 0 ? c.dxm : 0);
-        d += CY[o] > 0 ? c.dyp : (CY[o] < 0 ? c.dym : 0);
-        d += CZ[o] > 0 ? c.dzp : (CZ[o] < 0 ? c.dzm : 0);
-        j = (uint32_t)((int)i + d);
-    }
-    ST::store(a.p[plane], j, v, a.rest[WCLASS[q]]);
-}
-
-// Gather plane q of the current state into FP64 staging (0 on solids).
-template <class ST>
-__global__ __launch_bounds__(TPB) void k_gather(AuxArgs<ST> a)
-{
-    using R = typename ST::R;
-    int x, y, z;
-    bool ok;
-    const uint32_t i = aux_cell(a.g, x, y, z, ok);
-    if (!ok) return;
-    const uint8_t cls = a.cls[i];
-    if (cls == CL_SOLID) {
-        a.io[i] = 0.0;
-        return;
-    }
-    const Cell c = aux_neigh(a.g, x, y, z, i);
-    R f[Q];
-    cell_state(a, c, cls, f);
-    R v = f[0];
-#pragma unroll
-    for (int q = 1; q < Q; ++q)
-        if (q == a.q) v = f[q];
-    a.io[i] = (double)v;
-}
-
-// rho, u (and omega) of the current state into 4 (5) FP64 staging planes.
-template <class ST>
-__global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
-{
-    using R = typename ST::R;
-    int x, y, z;
-    bool ok;
-    const uint32_t i = aux_cell(a.g, x, y, z, ok);
-    if (!ok) return;
-    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    const uint8_t cls = a.cls[i];
-    if (cls == CL_SOLID) {
-        for (int k = 0; k < 5; ++k) a.io[k * N + i] = 0.0;
-        return;
-    }
-    const Cell c = aux_neigh(a.g, x, y, z, i);
-    R f[Q];
-    cell_state(a, c, cls, f);
-    R rho, ux, uy, uz, om;
-    if (a.lit) {
-        moments_literal(f, rho, ux, uy, uz);
-        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
-                                                  : a.omega0;
-    } else {
-        const Raw2<R> r = raw_moments(f);
-        rho = r.m0;
-        ux = r.jx / rho;
-        uy = r.jy / rho;
-        uz = r.jz / rho;
-        const Cen2<R> ce = central2(r, ux, uy, uz);
-        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
-    }
-    a.io[i] = (double)rho;
-    a.io[N + i] = (double)ux;
-    a.io[2 * N + i] = (double)uy;
-    a.io[3 * N + i] = (double)uz;
-    a.io[4 * N + i] = (double)om;
-}
-
-// rho, u of a single cell (probe).
-template <class ST>
-__global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
-{
-    using R = typename ST::R;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t i = (uint32_t)(x + a.g.nx * (y + a.g.ny * z));
-    const uint8_t cls = a.cls[i];
-    if (cls == CL_SOLID) {
-        for (int k = 0; k < 4; ++k) a.io[k] = 0.0;
-        return;
-    }
-    const Cell c = aux_neigh(a.g, x, y, z, i);
-    R f[Q];
-    cell_state(a, c, cls, f);
-    R rho, ux, uy, uz;
-    moments_literal(f, rho, ux, uy, uz);
-    a.io[0] = (double)rho;
-    a.io[1] = (double)ux;
-    a.io[2] = (double)uy;
-    a.io[3] = (double)uz;
-}
-
-// Per-block FP64 sums of mass and momentum over non-solid cells.
-template <class ST>
-__global__ __launch_bounds__(TPB) void k_sums(AuxArgs<ST> a)
-{
-    using R = typename ST::R;
-    int x, y, z;
-    bool ok;
-    const uint32_t i = aux_cell(a.g, x, y, z, ok);
-    double dm[4] = {0.0, 0.0, 0.0, 0.0};
-    if (ok && a.cls[i] != CL_SOLID) {
-        const Cell c = aux_neigh(a.g, x, y, z, i);
-        R f[Q];
-        cell_state(a, c, a.cls[i], f);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const double v = (double)f[q];
-            dm[0] += v;
-            dm[1] += CX[q] * v;
-            dm[2] += CY[q] * v;
-            dm[3] += CZ[q] * v;
-        }
-    }
-    __shared__ double red[TPB / 64][4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        double v = dm[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        dm[k] = v;
-    }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < 4; ++k) red[wv][k] = dm[k];
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double s = 0.0;
-        for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
-        a.io[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
-    }
-}
-
-// Plain stream copy used as the bandwidth reference.
-__global__ __launch_bounds__(TPB) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
-}  // namespace lbm
-
-// ===========================================================================
-// Host side
-// ===========================================================================
+#include "lbm_kernels.hpp"
 
 using namespace lbm;
 
@@ -609,20 +26,26 @@ struct lbm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t N = 0;
+    size_t N = 0;              // padded cells
+    size_t NU = 0;             // caller's cells
     size_t elem = 0;           // bytes per stored population
     size_t rbytes = 0;         // bytes per arithmetic value
     Geo g{};
-    bool flags_needed = false; // any non-fluid cell
-    int n_inlet = 0;
+    bool flags_needed = false; // any non-fluid cell (halo included)
+    int n_inlet = 0, n_eq = 0, n_outlet = 0, n_ghost = 0, n_pub = 0;
+    std::vector<int> ghost_off;  // launch boundaries of the ghost list (dependency levels)
     void* pops = nullptr;
     uint8_t* cls = nullptr;
     uint64_t* node = nullptr;
     int32_t* bcidx = nullptr;
     void* inl_state = nullptr;
-    InletDev* inl = nullptr;
-    void* macro = nullptr;     // 4 planes, arithmetic type
-    double* stage = nullptr;   // 5 planes FP64
+    NodeDev* inl = nullptr;
+    NodeDev* outl = nullptr;
+    NodeDev* ghost = nullptr;
+    void* pub_mac = nullptr;
+    void* pub_post = nullptr;
+    void* macro = nullptr;     // 4 planes, arithmetic type, padded layout
+    double* stage = nullptr;   // 5 planes FP64, caller layout
     double* partials = nullptr;
     std::vector<double> h_partials;
     int64_t steps = 0;
@@ -700,11 +123,59 @@ static AuxArgs<ST> aux_args(const lbm_ctx* c)
     a.closure = c->d.closure;
     a.lit = c->d.arith == LBM_ARITH_LITERAL;
     a.odd = (int)(c->steps & 1);
+    a.first = c->first ? 1 : 0;
     a.io = c->stage;
     return a;
 }
 
-static unsigned aux_blocks(const lbm_ctx* c) { return (unsigned)((c->N + TPB - 1) / TPB); }
+static unsigned blocks_for(size_t n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <class ST>
+static PatchArgs<ST> patch_args(const lbm_ctx* c)
+{
+    using R = typename ST::R;
+    PatchArgs<ST> a;
+    std::memset(&a, 0, sizeof(a));
+    fill_common<ST>(c, a.p, a.rest, a.K);
+    a.cls = c->cls;
+    a.bcidx = c->bcidx;
+    a.inl_state = (R*)c->inl_state;
+    a.pub_mac = (const R*)c->pub_mac;
+    a.pub_post = (const R*)c->pub_post;
+    a.g = c->g;
+    a.nu = (R)c->d.nu;
+    // the outlet's interpolation weight (OutletDensityInterpolated.h:226) is
+    // 1/sqrt(3) in the constants' precision
+    a.cs = (c->d.compat & LBM_COMPAT_F32CONST) ? (R)(1.f / sqrtf(3.f)) : (R)(1.0 / std::sqrt(3.0));
+    a.crev20 = (c->d.compat & LBM_COMPAT_CREV20) ? 1 : 0;
+    return a;
+}
+
+// Boundary rules that read what other cells produced in the same step, in
+// the reference's order: inlet, outlet, periodic.
+template <class ST>
+static lbm_status launch_patches(lbm_ctx* c, int odd_next)
+{
+    PatchArgs<ST> a = patch_args<ST>(c);
+    a.odd = odd_next;
+    if (c->n_eq) {
+        a.nodes = c->inl;
+        a.n = c->n_inlet;
+        k_inlet_eq<ST><<<blocks_for(a.n), TPB, 0, c->stream>>>(a);
+    }
+    if (c->n_outlet) {
+        a.nodes = c->outl;
+        a.n = c->n_outlet;
+        k_outlet<ST><<<blocks_for(a.n), TPB, 0, c->stream>>>(a);
+    }
+    for (size_t l = 0; l + 1 < c->ghost_off.size(); ++l) {
+        a.nodes = c->ghost + c->ghost_off[l];
+        a.n = c->ghost_off[l + 1] - c->ghost_off[l];
+        k_periodic<ST><<<blocks_for(a.n), TPB, 0, c->stream>>>(a);
+    }
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+}
 
 template <class ST, int COLL, bool LIT>
 static lbm_status launch_steps(lbm_ctx* c, int nsteps)
@@ -718,6 +189,8 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.bcidx = c->bcidx;
     a.inl_state = (R*)c->inl_state;
     a.inl = c->inl;
+    a.pub_mac = (R*)c->pub_mac;
+    a.pub_post = (R*)c->pub_post;
     a.macro = (const R*)c->macro;
     a.partials = c->partials;
     a.g = c->g;
@@ -726,6 +199,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.cles = (R)c->d.c_les;
     a.crev20 = (c->d.compat & LBM_COMPAT_CREV20) ? 1 : 0;
     a.closure = c->d.closure;
+    const bool patches = c->n_eq || c->n_outlet || c->n_ghost;
     const dim3 grid((unsigned)c->g.ntiles), block(TPB);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -740,6 +214,10 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
             else k_step<ST, COLL, LIT, false, false><<<grid, block, 0, c->stream>>>(a);
         }
         HIPCHK(hipGetLastError());
+        if (patches) {
+            lbm_status st = launch_patches<ST>(c, odd ? 0 : 1);
+            if (st != LBM_OK) return st;
+        }
         c->first = false;
         c->steps++;
     }
@@ -782,102 +260,315 @@ static lbm_status with_storage(const lbm_ctx* c, F&& fn)
 // Flag-grid classification
 // ---------------------------------------------------------------------------
 
-static bool neighbour(const lbm_desc* d, int x, int y, int z, int cx, int cy, int cz, size_t* out)
+namespace {
+
+struct Layout {
+    int ux, uy, uz;   // caller's lattice
+    int px, py, pz;   // halo per axis
+    int nx, ny, nz;   // padded
+    bool w[3];
+    size_t user(int x, int y, int z) const
+    {
+        return (size_t)x + (size_t)ux * ((size_t)y + (size_t)uy * (size_t)z);
+    }
+    size_t pad(int x, int y, int z) const
+    {
+        return (size_t)(x + px) + (size_t)nx * ((size_t)(y + py) + (size_t)ny * (size_t)(z + pz));
+    }
+    // neighbour (x+cx, y+cy, z+cz) on the caller's lattice, with index wrap
+    bool nbr(int x, int y, int z, int cx, int cy, int cz, int& ox, int& oy, int& oz) const
+    {
+        ox = x + cx;
+        oy = y + cy;
+        oz = z + cz;
+        if (ox < 0 || ox >= ux) { if (!w[0]) return false; ox = (ox + ux) % ux; }
+        if (oy < 0 || oy >= uy) { if (!w[1]) return false; oy = (oy + uy) % uy; }
+        if (oz < 0 || oz >= uz) { if (!w[2]) return false; oz = (oz + uz) % uz; }
+        return true;
+    }
+};
+
+enum Kind : uint8_t { K_SOLID, K_STREAM, K_INLET };
+
+struct Plan {
+    std::vector<uint8_t> cls;
+    std::vector<uint64_t> node;
+    std::vector<int32_t> bcidx;
+    std::vector<NodeDev> inl, outl, ghost;
+    std::vector<int> ghost_off;
+    int n_eq = 0;
+    int n_pub = 0;
+    bool flags_needed = false;
+};
+
+NodeDev node_dev(const Layout& L, const lbm_bc_node& b)
 {
-    int xx = x + cx, yy = y + cy, zz = z + cz;
-    if (xx < 0 || xx >= d->nx) { if (!d->periodic[0]) return false; xx = (xx + d->nx) % d->nx; }
-    if (yy < 0 || yy >= d->ny) { if (!d->periodic[1]) return false; yy = (yy + d->ny) % d->ny; }
-    if (zz < 0 || zz >= d->nz) { if (!d->periodic[2]) return false; zz = (zz + d->nz) % d->nz; }
-    *out = (size_t)xx + (size_t)d->nx * ((size_t)yy + (size_t)d->ny * (size_t)zz);
-    return true;
+    NodeDev n;
+    n.x = b.x + L.px;
+    n.y = b.y + L.py;
+    n.z = b.z + L.pz;
+    n.kind = b.kind;
+    n.nx = b.nx;
+    n.ny = b.ny;
+    n.nz = b.nz;
+    n.ux = b.ux;
+    n.uy = b.uy;
+    n.uz = b.uz;
+    n.rho = b.rho;
+    n.regular = b.regular;
+    n.s0 = n.s1 = -1;
+    n.partner = -1;
+    return n;
 }
 
-static lbm_status classify(const lbm_desc* d, std::vector<uint8_t>& cls, std::vector<uint64_t>& node,
-                           std::vector<int32_t>& bcidx, std::vector<InletDev>& inl, bool& flags_needed)
+}  // namespace
+
+static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
 {
-    const size_t N = (size_t)d->nx * d->ny * d->nz;
-    cls.assign(N, CL_SOLID);
-    node.assign(N, 0);
-    bcidx.assign(N, -1);
-    inl.clear();
-    std::vector<uint8_t> wall(N, 0);
+    const size_t NU = (size_t)L.ux * L.uy * L.uz;
+    const size_t N = (size_t)L.nx * L.ny * L.nz;
+    std::vector<uint8_t> kind(NU);
+    for (size_t i = 0; i < NU; ++i) {
+        const int F = d->flags[i];
+        if (F == 0) kind[i] = K_SOLID;
+        else if (F > 0 || F == -2 || F == -3) kind[i] = K_STREAM;   // StreamingAB.h:49
+        else if (F == -1) kind[i] = K_INLET;
+        else return LBM_E_UNSUPPORTED;   // other negative flags (symmetry planes) are not built here
+    }
+    std::vector<uint8_t> wall(NU, 0);
+    std::vector<int32_t> inlet_of(NU, -1), pub_of(NU, -1);
+    std::vector<uint8_t> has_outlet(NU, 0);
+    P.inl.clear();
+    P.outl.clear();
+    P.ghost.clear();
+    P.n_eq = 0;
+    P.n_pub = 0;
+    auto pub_slot = [&](size_t iu) -> int32_t {
+        if (pub_of[iu] < 0) pub_of[iu] = P.n_pub++;
+        return pub_of[iu];
+    };
+    // slot of a neighbour whose rho/u/f* a patch reads: -1 for solid cells
+    // (zero in the reference), error for inlet cells (not published here)
+    auto ref_slot = [&](int x, int y, int z, int32_t& slot) -> lbm_status {
+        const size_t j = L.user(x, y, z);
+        if (kind[j] == K_SOLID) slot = -1;
+        else if (kind[j] == K_INLET) return LBM_E_UNSUPPORTED;
+        else slot = pub_slot(j);
+        return LBM_OK;
+    };
+
     for (int n = 0; n < d->n_bc; ++n) {
         const lbm_bc_node& b = d->bc[n];
-        if (b.x < 0 || b.y < 0 || b.z < 0 || b.x >= d->nx || b.y >= d->ny || b.z >= d->nz) return LBM_E_ARG;
-        const size_t i = (size_t)b.x + (size_t)d->nx * ((size_t)b.y + (size_t)d->ny * (size_t)b.z);
+        if (b.x < 0 || b.y < 0 || b.z < 0 || b.x >= L.ux || b.y >= L.uy || b.z >= L.uz) return LBM_E_ARG;
+        const size_t i = L.user(b.x, b.y, b.z);
         const int F = d->flags[i];
         switch (b.kind) {
         case LBM_BC_WALL:
             if (F <= 0) return LBM_E_ARG;  // wall list holds fluid nodes only (geometryMesherBoundary.h:540-542)
             wall[i] = 1;
             break;
-        case LBM_BC_INLET_MOVING_WALL: {
-            if (F != -1 || bcidx[i] >= 0) return LBM_E_ARG;
-            InletDev e{b.nx, b.ny, b.nz, b.ux, b.uy, b.uz, b.kind, 0};
-            bcidx[i] = (int32_t)inl.size();
-            inl.push_back(e);
+        case LBM_BC_INLET_MOVING_WALL:
+        case LBM_BC_INLET_EQUILIBRIUM: {
+            if (F != -1 || inlet_of[i] >= 0) return LBM_E_ARG;
+            NodeDev e = node_dev(L, b);
+            if (b.kind == LBM_BC_INLET_EQUILIBRIUM) {
+                // neighbours x - n and x - 2n along +-x (InletVelocityEquilibrium.h:58-97)
+                const int sx = (b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f) ? 1
+                             : (b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f) ? -1 : 0;
+                int x1, y1, z1, x2, y2, z2;
+                if (sx != 0 && L.nbr(b.x, b.y, b.z, -sx, 0, 0, x1, y1, z1) &&
+                    L.nbr(b.x, b.y, b.z, -2 * sx, 0, 0, x2, y2, z2)) {
+                    lbm_status st = ref_slot(x1, y1, z1, e.s0);
+                    if (st == LBM_OK) st = ref_slot(x2, y2, z2, e.s1);
+                    if (st != LBM_OK) return st;
+                    P.n_eq++;
+                } else {
+                    e.kind = -1;   // the reference applies nothing to this node
+                }
+            }
+            inlet_of[i] = (int32_t)P.inl.size();
+            P.inl.push_back(e);
             break;
         }
-        case LBM_BC_INLET_EQUILIBRIUM:
-        case LBM_BC_OUTLET_INTERPOLATED:
-        case LBM_BC_PERIODIC_GHOST:
-            return LBM_E_UNSUPPORTED;
+        case LBM_BC_OUTLET_INTERPOLATED: {
+            if (kind[i] == K_SOLID) return LBM_E_ARG;
+            if (kind[i] == K_INLET || has_outlet[i]) return LBM_E_UNSUPPORTED;
+            has_outlet[i] = 1;
+            const int sx = (b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f) ? 1
+                         : (b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f) ? -1 : 0;
+            if (sx == 0) break;   // other normals: no action (OutletDensityInterpolated.h:249)
+            NodeDev e = node_dev(L, b);
+            e.s0 = pub_slot(i);
+            int xu, yu, zu;
+            if (L.nbr(b.x, b.y, b.z, -sx, 0, 0, xu, yu, zu)) {
+                lbm_status st = ref_slot(xu, yu, zu, e.s1);
+                if (st != LBM_OK) return st;
+            } else {
+                e.s1 = -2;   // only the irregular-node copy applies
+            }
+            P.outl.push_back(e);
+            break;
+        }
+        case LBM_BC_PERIODIC_GHOST: {
+            if (kind[i] == K_SOLID) return LBM_E_ARG;
+            if (kind[i] == K_INLET) return LBM_E_UNSUPPORTED;
+            const float ax = std::fabs(b.nx), ay = std::fabs(b.ny), az = std::fabs(b.nz);
+            const int axis = (ax == 1.0f && ay == 0.0f && az == 0.0f) ? 0
+                           : (ax == 0.0f && ay == 1.0f && az == 0.0f) ? 1
+                           : (ax == 0.0f && ay == 0.0f && az == 1.0f) ? 2 : -1;
+            if (axis < 0) break;   // no action (Periodic.h:136)
+            const int ext = axis == 0 ? L.ux : axis == 1 ? L.uy : L.uz;
+            if (b.per_idx < 0 || b.per_idx >= ext) return LBM_E_ARG;
+            NodeDev e = node_dev(L, b);
+            const int qx = axis == 0 ? b.per_idx : b.x, qy = axis == 1 ? b.per_idx : b.y,
+                      qz = axis == 2 ? b.per_idx : b.z;
+            e.partner = (int32_t)L.pad(qx, qy, qz);
+            e.s0 = b.regular ? -1 : pub_slot(i);
+            P.ghost.push_back(e);
+            break;
+        }
         default:
             return LBM_E_ARG;
         }
     }
-    // cell classes
-    for (size_t i = 0; i < N; ++i) {
-        const int F = d->flags[i];
-        if (F == 0) cls[i] = CL_SOLID;
-        else if (F > 0) cls[i] = CL_FLUID;
-        else if (F == -1) {
-            if (bcidx[i] < 0) return LBM_E_ARG;  // inlet flag without an inlet entry
-            cls[i] = CL_INLET_MW;
-        } else {
-            return LBM_E_UNSUPPORTED;
+    // flag -1 cells without an inlet entry keep their state and stream it
+    for (size_t i = 0; i < NU; ++i) {
+        if (kind[i] == K_INLET && inlet_of[i] < 0) {
+            NodeDev e{};
+            const int x = (int)(i % L.ux), y = (int)((i / L.ux) % L.uy), z = (int)(i / ((size_t)L.ux * L.uy));
+            e.x = x + L.px;
+            e.y = y + L.py;
+            e.z = z + L.pz;
+            e.kind = -1;
+            e.s0 = e.s1 = e.partner = -1;
+            inlet_of[i] = (int32_t)P.inl.size();
+            P.inl.push_back(e);
         }
     }
-    flags_needed = false;
-    for (int z = 0; z < d->nz; ++z)
-        for (int y = 0; y < d->ny; ++y)
-            for (int x = 0; x < d->nx; ++x) {
-                const size_t i = (size_t)x + (size_t)d->nx * ((size_t)y + (size_t)d->ny * (size_t)z);
-                if (cls[i] == CL_SOLID) { flags_needed = true; continue; }
+
+    // Ghost entries act in list order (Periodic.h:50-138): an entry must run
+    // after every earlier entry that writes its cell or its partner, or reads
+    // its cell as a partner.  Entries of one level are independent.
+    if (!P.ghost.empty()) {
+        std::unordered_map<int64_t, int> last_write, last_read;
+        std::vector<int> level(P.ghost.size());
+        int max_level = 0;
+        for (size_t k = 0; k < P.ghost.size(); ++k) {
+            const NodeDev& e = P.ghost[k];
+            const int64_t self = (int64_t)e.x + (int64_t)L.nx * ((int64_t)e.y + (int64_t)L.ny * e.z);
+            const int64_t part = e.partner;
+            int lv = 0;
+            auto bump = [&](std::unordered_map<int64_t, int>& m, int64_t key) {
+                auto it = m.find(key);
+                if (it != m.end()) lv = std::max(lv, it->second + 1);
+            };
+            bump(last_write, self);
+            bump(last_write, part);
+            bump(last_read, self);
+            level[k] = lv;
+            last_write[self] = std::max(last_write.count(self) ? last_write[self] : 0, lv);
+            if (part != self) last_read[part] = std::max(last_read.count(part) ? last_read[part] : 0, lv);
+            max_level = std::max(max_level, lv);
+        }
+        std::vector<NodeDev> sorted;
+        sorted.reserve(P.ghost.size());
+        P.ghost_off.assign(1, 0);
+        for (int lv = 0; lv <= max_level; ++lv) {
+            for (size_t k = 0; k < P.ghost.size(); ++k)
+                if (level[k] == lv) sorted.push_back(P.ghost[k]);
+            P.ghost_off.push_back((int)sorted.size());
+        }
+        P.ghost.swap(sorted);
+    } else {
+        P.ghost_off.clear();
+    }
+
+    // cell classes and stream masks on the padded lattice
+    P.cls.assign(N, CL_SOLID);
+    P.node.assign(N, 0);
+    P.bcidx.assign(N, -1);
+    P.flags_needed = N != NU;
+    for (int z = 0; z < L.uz; ++z)
+        for (int y = 0; y < L.uy; ++y)
+            for (int x = 0; x < L.ux; ++x) {
+                const size_t iu = L.user(x, y, z), ip = L.pad(x, y, z);
+                if (kind[iu] == K_SOLID) {
+                    P.flags_needed = true;
+                    continue;
+                }
                 uint32_t S = 0, K = 0;
                 for (int q = 1; q < Q; ++q) {
-                    size_t j;
-                    if (!neighbour(d, x, y, z, -CX[q], -CY[q], -CZ[q], &j)) return LBM_E_ARG;
-                    if (cls[j] == CL_SOLID) S |= 1u << q;
-                    if (!neighbour(d, x, y, z, CX[q], CY[q], CZ[q], &j)) return LBM_E_ARG;
-                    if (cls[j] == CL_SOLID || cls[j] == CL_INLET_MW) K |= 1u << q;
+                    int ox, oy, oz;
+                    // source x - c_q: solid -> bounce-back (wall nodes) or untouched;
+                    // outside the lattice -> untouched
+                    if (!L.nbr(x, y, z, -CX[q], -CY[q], -CZ[q], ox, oy, oz)) {
+                        if (wall[iu]) return LBM_E_UNSUPPORTED;
+                        S |= 1u << q;
+                    } else if (kind[L.user(ox, oy, oz)] == K_SOLID) {
+                        S |= 1u << q;
+                    }
+                    // destination x + c_q: solid, inlet or outside -> no streaming write
+                    if (!L.nbr(x, y, z, CX[q], CY[q], CZ[q], ox, oy, oz) ||
+                        kind[L.user(ox, oy, oz)] != K_STREAM)
+                        K |= 1u << q;
                 }
-                uint64_t w = (uint64_t)S | ((uint64_t)K << 32) | ((uint64_t)wall[i] << 63);
-                node[i] = w;
-                if (cls[i] == CL_INLET_MW) { flags_needed = true; continue; }
-                if (S != 0 || K != 0) {
-                    cls[i] = CL_BND;
-                    flags_needed = true;
+                P.node[ip] = (uint64_t)S | ((uint64_t)K << 32) | ((uint64_t)wall[iu] << 63);
+                if (kind[iu] == K_INLET) {
+                    P.cls[ip] = CL_INLET;
+                    P.bcidx[ip] = inlet_of[iu];
+                    P.flags_needed = true;
+                } else if (pub_of[iu] >= 0) {
+                    P.cls[ip] = CL_PUB;
+                    P.bcidx[ip] = pub_of[iu];
+                    P.flags_needed = true;
+                } else if (S != 0 || K != 0 || wall[iu]) {
+                    P.cls[ip] = CL_BND;
+                    P.flags_needed = true;
+                } else {
+                    P.cls[ip] = CL_FLUID;
                 }
             }
     return LBM_OK;
 }
 
-// Upload host FP64 planes into the arithmetic-type macro buffer.
+// Upload host FP64 planes (caller layout) into the padded macro buffer.
 template <class R>
 static lbm_status upload_macro(lbm_ctx* c, const double* rho, const double* ux, const double* uy, const double* uz)
 {
     const size_t N = c->N;
-    std::vector<R> h(4 * N);
-    for (size_t i = 0; i < N; ++i) {
-        h[i] = rho ? (R)rho[i] : (R)c->d.rho0;
-        h[N + i] = ux ? (R)ux[i] : R(0);
-        h[2 * N + i] = uy ? (R)uy[i] : R(0);
-        h[3 * N + i] = uz ? (R)uz[i] : R(0);
-    }
+    std::vector<R> h(4 * N, R(0));
+    const Geo& g = c->g;
+    size_t iu = 0;
+    for (int z = 0; z < g.uz; ++z)
+        for (int y = 0; y < g.uy; ++y)
+            for (int x = 0; x < g.ux; ++x, ++iu) {
+                const size_t i = (size_t)(x + g.px) + (size_t)g.nx * ((size_t)(y + g.py) + (size_t)g.ny * (z + g.pz));
+                h[i] = rho ? (R)rho[iu] : (R)c->d.rho0;
+                h[N + i] = ux ? (R)ux[iu] : R(0);
+                h[2 * N + i] = uy ? (R)uy[iu] : R(0);
+                h[3 * N + i] = uz ? (R)uz[iu] : R(0);
+            }
     HIPCHK(hipMemcpyAsync(c->macro, h.data(), 4 * N * sizeof(R), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
+}
+
+// Tile width 2^bxl along x: the widest whose padding waste stays under 15%,
+// else the one with the least waste.
+static int pick_bx_log2(int nx)
+{
+    int best = 8;
+    double best_util = 0.0;
+    for (int b = 8; b >= 5; --b) {
+        const int w = 1 << b;
+        const double util = (double)nx / (double)(((nx + w - 1) / w) * w);
+        if (util >= 0.85) return b;
+        if (util > best_util) {
+            best_util = util;
+            best = b;
+        }
+    }
+    return best;
 }
 
 // ---------------------------------------------------------------------------
@@ -903,7 +594,8 @@ const char* lbm_status_str(lbm_status s)
 void lbm_destroy(lbm_ctx* c)
 {
     if (!c) return;
-    void* bufs[] = {c->pops, c->cls, c->node, c->bcidx, c->inl_state, c->inl, c->macro, c->stage, c->partials};
+    void* bufs[] = {c->pops,  c->cls,     c->node,     c->bcidx, c->inl_state, c->inl,     c->outl,
+                    c->ghost, c->pub_mac, c->pub_post, c->macro, c->stage,     c->partials};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -927,19 +619,30 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     // tau > 1/2 keeps omega in (0, 2); NonDim...h:79 asserts this (compiled out by -DNDEBUG)
     if (!(d->tau > 0.5) || !std::isfinite(d->tau)) return LBM_E_ARG;
     if (d->closure == LBM_CLOSURE_SMAGORINSKY && !(d->c_les >= 0.0)) return LBM_E_ARG;
-    const size_t N = (size_t)d->nx * d->ny * d->nz;
-    const size_t elem = d->storage == LBM_STORAGE_F64 ? 8 : d->storage == LBM_STORAGE_F32 ? 4 : 2;
-    if (N * elem > 0xFFFFFFFFull || N >= 0x7FFFFFFFull) return LBM_E_ARG;  // 32-bit plane offsets
     if (d->periodic[0] && d->nx < 2) return LBM_E_ARG;
     if (d->periodic[1] && d->ny < 2) return LBM_E_ARG;
     if (d->periodic[2] && d->nz < 2) return LBM_E_ARG;
 
-    std::vector<uint8_t> cls;
-    std::vector<uint64_t> node;
-    std::vector<int32_t> bcidx;
-    std::vector<InletDev> inl;
-    bool flags_needed = false;
-    lbm_status st = classify(d, cls, node, bcidx, inl, flags_needed);
+    Layout L;
+    L.ux = d->nx;
+    L.uy = d->ny;
+    L.uz = d->nz;
+    L.w[0] = d->periodic[0] != 0;
+    L.w[1] = d->periodic[1] != 0;
+    L.w[2] = d->periodic[2] != 0;
+    L.px = L.w[0] ? 0 : 1;
+    L.py = L.w[1] ? 0 : 1;
+    L.pz = L.w[2] ? 0 : 1;
+    L.nx = L.ux + 2 * L.px;
+    L.ny = L.uy + 2 * L.py;
+    L.nz = L.uz + 2 * L.pz;
+    const size_t N = (size_t)L.nx * L.ny * L.nz;
+    const size_t NU = (size_t)L.ux * L.uy * L.uz;
+    const size_t elem = d->storage == LBM_STORAGE_F64 ? 8 : d->storage == LBM_STORAGE_F32 ? 4 : 2;
+    if (N * elem > 0xFFFFFFFFull || N >= 0x7FFFFFFFull) return LBM_E_ARG;  // 32-bit plane offsets
+
+    Plan P;
+    lbm_status st = classify(d, L, P);
     if (st != LBM_OK) return st;
 
     int ndev = 0;
@@ -954,10 +657,16 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     c->d.flags = nullptr;
     c->d.bc = nullptr;
     c->N = N;
+    c->NU = NU;
     c->elem = elem;
     c->rbytes = d->storage == LBM_STORAGE_F64 ? 8 : 4;
-    c->flags_needed = flags_needed;
-    c->n_inlet = (int)inl.size();
+    c->flags_needed = P.flags_needed;
+    c->n_inlet = (int)P.inl.size();
+    c->n_eq = P.n_eq;
+    c->n_outlet = (int)P.outl.size();
+    c->n_ghost = (int)P.ghost.size();
+    c->n_pub = P.n_pub;
+    c->ghost_off = P.ghost_off;
     (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -965,43 +674,62 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
         return LBM_E_HIP;
     }
     // tile geometry: 256 threads = (1 << bx_log2) x by cells
-    int bxl = 6;
-    if (d->nx > 64) bxl = 7;
-    if (d->nx > 128) bxl = 8;
-    c->g.nx = d->nx;
-    c->g.ny = d->ny;
-    c->g.nz = d->nz;
+    const int bxl = pick_bx_log2(L.nx);
+    c->g.nx = L.nx;
+    c->g.ny = L.ny;
+    c->g.nz = L.nz;
+    c->g.px = L.px;
+    c->g.py = L.py;
+    c->g.pz = L.pz;
+    c->g.ux = L.ux;
+    c->g.uy = L.uy;
+    c->g.uz = L.uz;
     c->g.bx_log2 = bxl;
     c->g.by = TPB >> bxl;
-    c->g.ntx = (d->nx + (1 << bxl) - 1) >> bxl;
-    c->g.nty = (d->ny + c->g.by - 1) / c->g.by;
-    c->g.ntiles = c->g.ntx * c->g.nty * d->nz;
-    c->g.wx = d->periodic[0] != 0;
-    c->g.wy = d->periodic[1] != 0;
-    c->g.wz = d->periodic[2] != 0;
+    c->g.ntx = (L.nx + (1 << bxl) - 1) >> bxl;
+    c->g.nty = (L.ny + c->g.by - 1) / c->g.by;
+    c->g.ntiles = c->g.ntx * c->g.nty * L.nz;
+    c->g.wx = L.w[0];
+    c->g.wy = L.w[1];
+    c->g.wz = L.w[2];
 
+    const size_t rb = c->rbytes;
     st = dev_alloc(c, &c->pops, N * elem * Q);
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->cls, N);
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->node, N * sizeof(uint64_t));
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->bcidx, N * sizeof(int32_t));
-    if (st == LBM_OK) st = dev_alloc(c, &c->inl_state, (size_t)c->n_inlet * Q * c->rbytes);
-    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->inl, inl.size() * sizeof(InletDev));
-    if (st == LBM_OK) st = dev_alloc(c, &c->macro, 4 * N * c->rbytes);
-    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->stage, 5 * N * sizeof(double));
-    const size_t nparts = std::max((size_t)c->g.ntiles, (size_t)aux_blocks(c));
+    if (st == LBM_OK) st = dev_alloc(c, &c->inl_state, (size_t)c->n_inlet * Q * rb);
+    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->inl, P.inl.size() * sizeof(NodeDev));
+    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->outl, P.outl.size() * sizeof(NodeDev));
+    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->ghost, P.ghost.size() * sizeof(NodeDev));
+    if (st == LBM_OK) st = dev_alloc(c, &c->pub_mac, (size_t)c->n_pub * 2 * rb);
+    if (st == LBM_OK) st = dev_alloc(c, &c->pub_post, (size_t)c->n_pub * Q * rb);
+    if (st == LBM_OK) st = dev_alloc(c, &c->macro, 4 * N * rb);
+    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->stage, 5 * NU * sizeof(double));
+    const size_t nparts = std::max((size_t)c->g.ntiles, (size_t)blocks_for(NU));
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->partials, nparts * 4 * sizeof(double));
     if (st != LBM_OK) {
         lbm_destroy(c);
         return st;
     }
     c->h_partials.resize(nparts * 4);
-    if (hipMemcpyAsync(c->cls, cls.data(), N, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->node, node.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->bcidx, bcidx.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        (!inl.empty() && hipMemcpyAsync(c->inl, inl.data(), inl.size() * sizeof(InletDev), hipMemcpyHostToDevice,
-                                        c->stream) != hipSuccess) ||
-        hipMemsetAsync(c->pops, 0, N * elem * Q, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    };
+    bool ok = up(c->cls, P.cls.data(), N) && up(c->node, P.node.data(), N * sizeof(uint64_t)) &&
+              up(c->bcidx, P.bcidx.data(), N * sizeof(int32_t)) &&
+              up(c->inl, P.inl.data(), P.inl.size() * sizeof(NodeDev)) &&
+              up(c->outl, P.outl.data(), P.outl.size() * sizeof(NodeDev)) &&
+              up(c->ghost, P.ghost.data(), P.ghost.size() * sizeof(NodeDev));
+    ok = ok && hipMemsetAsync(c->pops, 0, N * elem * Q, c->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(c->inl_state, 0, std::max<size_t>(16, (size_t)c->n_inlet * Q * rb), c->stream) ==
+                   hipSuccess;
+    ok = ok && hipMemsetAsync(c->pub_mac, 0, std::max<size_t>(16, (size_t)c->n_pub * 2 * rb), c->stream) ==
+                   hipSuccess;
+    ok = ok && hipMemsetAsync(c->pub_post, 0, std::max<size_t>(16, (size_t)c->n_pub * Q * rb), c->stream) ==
+                   hipSuccess;
+    ok = ok && hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) {
         lbm_destroy(c);
         return LBM_E_HIP;
     }
@@ -1021,7 +749,7 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* ux,
         using ST = decltype(stv);
         AuxArgs<ST> a = aux_args<ST>(c);
         a.odd = 0;
-        k_init<ST><<<aux_blocks(c), TPB, 0, c->stream>>>(a);
+        k_init<ST><<<blocks_for(c->N), TPB, 0, c->stream>>>(a);
         HIPCHK(hipGetLastError());
         return LBM_OK;
     });
@@ -1037,13 +765,13 @@ lbm_status lbm_set_populations(lbm_ctx* c, const double* f)
 {
     if (!c || !f) return LBM_E_ARG;
     for (int q = 0; q < Q; ++q) {
-        HIPCHK(hipMemcpyAsync(c->stage, f + (size_t)q * c->N, c->N * sizeof(double), hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(c->stage, f + (size_t)q * c->NU, c->NU * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
         lbm_status st = with_storage(c, [&](auto stv) -> lbm_status {
             using ST = decltype(stv);
             AuxArgs<ST> a = aux_args<ST>(c);
             a.q = q;
-            k_scatter<ST><<<aux_blocks(c), TPB, 0, c->stream>>>(a);
+            k_scatter<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
             HIPCHK(hipGetLastError());
             return LBM_OK;
         });
@@ -1086,44 +814,31 @@ lbm_status lbm_sync(lbm_ctx* c)
     return c->nan_seen ? LBM_E_NAN : LBM_OK;
 }
 
+// rho, u, omega of the current state into the 5 staging planes.
+static lbm_status run_macro(lbm_ctx* c)
+{
+    return with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        k_macro<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+}
+
 lbm_status lbm_get_macroscopic(lbm_ctx* c, double* rho, double* ux, double* uy, double* uz)
 {
     if (!c) return LBM_E_ARG;
     if (!c->initialized) return LBM_E_STATE;
     lbm_status st = lbm_sync(c);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    const size_t N = c->N;
-    double* outs[4] = {rho, ux, uy, uz};
-    if (c->first) {
-        // before the first step the reference's rho/u arrays hold the initial values
-        std::vector<uint8_t> cls(N);
-        HIPCHK(hipMemcpy(cls.data(), c->cls, N, hipMemcpyDeviceToHost));
-        if (c->rbytes == 8) {
-            std::vector<double> h(4 * N);
-            HIPCHK(hipMemcpy(h.data(), c->macro, 4 * N * sizeof(double), hipMemcpyDeviceToHost));
-            for (int k = 0; k < 4; ++k)
-                if (outs[k])
-                    for (size_t i = 0; i < N; ++i) outs[k][i] = cls[i] == CL_SOLID ? 0.0 : h[k * N + i];
-        } else {
-            std::vector<float> h(4 * N);
-            HIPCHK(hipMemcpy(h.data(), c->macro, 4 * N * sizeof(float), hipMemcpyDeviceToHost));
-            for (int k = 0; k < 4; ++k)
-                if (outs[k])
-                    for (size_t i = 0; i < N; ++i) outs[k][i] = cls[i] == CL_SOLID ? 0.0 : (double)h[k * N + i];
-        }
-        return LBM_OK;
-    }
-    st = with_storage(c, [&](auto stv) -> lbm_status {
-        using ST = decltype(stv);
-        AuxArgs<ST> a = aux_args<ST>(c);
-        k_macro<ST><<<aux_blocks(c), TPB, 0, c->stream>>>(a);
-        HIPCHK(hipGetLastError());
-        return LBM_OK;
-    });
+    st = run_macro(c);
     if (st != LBM_OK) return st;
+    double* outs[4] = {rho, ux, uy, uz};
     for (int k = 0; k < 4; ++k)
         if (outs[k])
-            HIPCHK(hipMemcpyAsync(outs[k], c->stage + k * N, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(outs[k], c->stage + k * c->NU, c->NU * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
@@ -1131,27 +846,10 @@ lbm_status lbm_get_macroscopic(lbm_ctx* c, double* rho, double* ux, double* uy, 
 lbm_status lbm_probe(lbm_ctx* c, int32_t x, int32_t y, int32_t z, double out[4])
 {
     if (!c || !out) return LBM_E_ARG;
-    if (x < 0 || y < 0 || z < 0 || x >= c->g.nx || y >= c->g.ny || z >= c->g.nz) return LBM_E_ARG;
+    if (x < 0 || y < 0 || z < 0 || x >= c->g.ux || y >= c->g.uy || z >= c->g.uz) return LBM_E_ARG;
     if (!c->initialized) return LBM_E_STATE;
     lbm_status st = lbm_sync(c);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    if (c->first) {
-        double tmp[4];
-        const size_t i = (size_t)x + (size_t)c->g.nx * ((size_t)y + (size_t)c->g.ny * (size_t)z);
-        uint8_t cls = 0;
-        HIPCHK(hipMemcpy(&cls, c->cls + i, 1, hipMemcpyDeviceToHost));
-        for (int k = 0; k < 4; ++k) {
-            if (c->rbytes == 8) {
-                HIPCHK(hipMemcpy(&tmp[k], (double*)c->macro + k * c->N + i, 8, hipMemcpyDeviceToHost));
-            } else {
-                float v;
-                HIPCHK(hipMemcpy(&v, (float*)c->macro + k * c->N + i, 4, hipMemcpyDeviceToHost));
-                tmp[k] = v;
-            }
-            out[k] = cls == CL_SOLID ? 0.0 : tmp[k];
-        }
-        return LBM_OK;
-    }
     st = with_storage(c, [&](auto stv) -> lbm_status {
         using ST = decltype(stv);
         AuxArgs<ST> a = aux_args<ST>(c);
@@ -1171,15 +869,9 @@ lbm_status lbm_get_omega(lbm_ctx* c, double* omega)
     if (!c->initialized) return LBM_E_STATE;
     lbm_status st = lbm_sync(c);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    st = with_storage(c, [&](auto stv) -> lbm_status {
-        using ST = decltype(stv);
-        AuxArgs<ST> a = aux_args<ST>(c);
-        k_macro<ST><<<aux_blocks(c), TPB, 0, c->stream>>>(a);
-        HIPCHK(hipGetLastError());
-        return LBM_OK;
-    });
+    st = run_macro(c);
     if (st != LBM_OK) return st;
-    HIPCHK(hipMemcpyAsync(omega, c->stage + 4 * c->N, c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(omega, c->stage + 4 * c->NU, c->NU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
@@ -1195,12 +887,12 @@ lbm_status lbm_get_populations(lbm_ctx* c, double* f)
             using ST = decltype(stv);
             AuxArgs<ST> a = aux_args<ST>(c);
             a.q = q;
-            k_gather<ST><<<aux_blocks(c), TPB, 0, c->stream>>>(a);
+            k_gather<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
             HIPCHK(hipGetLastError());
             return LBM_OK;
         });
         if (st != LBM_OK) return st;
-        HIPCHK(hipMemcpyAsync(f + (size_t)q * c->N, c->stage, c->N * sizeof(double), hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(f + (size_t)q * c->NU, c->stage, c->NU * sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1213,7 +905,7 @@ lbm_status lbm_conservation(lbm_ctx* c, double out[4])
     if (!c->initialized) return LBM_E_STATE;
     lbm_status st = lbm_sync(c);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    const unsigned nb = aux_blocks(c);
+    const unsigned nb = blocks_for(c->NU);
     st = with_storage(c, [&](auto stv) -> lbm_status {
         using ST = decltype(stv);
         AuxArgs<ST> a = aux_args<ST>(c);

@@ -1,0 +1,847 @@
+// lbm_kernels.hpp -- device code of the collide-stream core (gfx950).
+//
+// One fused kernel per time step (k_step): load the 27 populations of a cell,
+// derive rho/u, the relaxation rate (constant or Smagorinsky), collide
+// (cumulant 2015 or BGK) and stream in place with the A-A pattern, applying
+// bounce-back and the moving-wall inlet from the flag grid in the same pass.
+// It replaces the reference's per-step operator chain
+// OmegaLES -> Collision -> StreamingAB -> BounceBackWallHalf -> Inlet ->
+// Outlet -> Periodic -> MomentDensityVelocityN27 (SolverTurbulentLES.h:187-214),
+// which reads and writes two full lattices plus rho/u/omega every step.
+//
+// A-A layout: one population array P[q][cell] (structure of arrays, x
+// fastest, one-cell solid halo on every non-periodic axis).  With t the
+// number of completed steps:
+//   even t: cell x reads f_q from P[q][x] and writes post-collision f*_q to
+//           P[opp q][x];
+//   odd t:  cell x reads f_q from P[opp q][x - c_q] and writes f*_q to
+//           P[q][x + c_q].
+// Every slot is read and written by the same thread within a step, so the
+// scheme is race-free without a second lattice.
+//
+// Boundaries (reference semantics; the oracle restates the AB step):
+//  * source x - c_q solid: the cell writes the slot that source would have
+//    written -- f*_{crev q} for wall-list nodes (half-way bounce-back,
+//    BounceBackWallHalf.h:46-47) or its own pre-collision f_q otherwise
+//    (StreamingAB.h:58-65 leaves the slot untouched);
+//  * destination x + c_q solid or inlet: the streaming write is skipped;
+//  * inlet cells keep their own state in a side buffer and stream their
+//    pre-collision populations (StreamingAB.h:62-65); the moving-wall rule
+//    (InletVelocityMovingWall.h:43-49) updates that state in the same pass;
+//  * the equilibrium inlet, interpolated outlet and ghost-layer periodic
+//    rules read values that other cells produce in the same step, so the
+//    fused kernel publishes them (neighbour rho/u, post-collision populations)
+//    into small side buffers and per-list patch kernels apply the rules after
+//    the step, in the reference's order (inlet, outlet, periodic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/lbm_core.h"
+#include "collide.hpp"
+#include "storage.hpp"
+
+namespace lbm {
+
+// per-cell class
+enum : uint8_t {
+    CL_SOLID = 0,
+    CL_FLUID = 1,  // streamed, every neighbour streamed too
+    CL_BND = 2,    // streamed, some neighbour solid, inlet or outside (masks in the node word)
+    CL_INLET = 3,  // inlet node (flag -1): own state in a side buffer
+    CL_PUB = 4     // streamed like CL_BND; also publishes rho, u_x and f* for a patch kernel
+};
+
+constexpr int TPB = 256;
+constexpr uint64_t MASK27 = (1ull << 27) - 1;
+
+struct Geo {
+    int nx, ny, nz;     // padded lattice
+    int px, py, pz;     // halo width per axis (0 on index-wrap axes)
+    int ux, uy, uz;     // caller's lattice
+    int bx_log2, by;    // tile: (1 << bx_log2) x by cells, 256 threads
+    int ntx, nty;       // tiles along x and y
+    int ntiles;
+    int wx, wy, wz;     // index wrap per axis
+};
+
+// Boundary-node record on the device (padded coordinates).  Publish slots
+// index the per-step side buffers written by CL_PUB cells; -1 stands for a
+// solid cell, whose rho, u and f* the reference keeps at zero.
+struct NodeDev {
+    int32_t x, y, z;
+    int32_t kind;      // LBM_BC_*
+    float nx, ny, nz;
+    float ux, uy, uz;
+    float rho;
+    int32_t regular;
+    int32_t s0;        // equilibrium inlet: slot of x - n; outlet and ghost: own slot
+    int32_t s1;        // equilibrium inlet: slot of x - 2n; outlet: upstream slot, -2 when outside the lattice
+    int32_t partner;   // ghost: padded index of the partner cell
+};
+
+template <class ST>
+struct StepArgs {
+    typename ST::T* p[Q];
+    const uint8_t* cls;
+    const uint64_t* node;
+    const int32_t* bcidx;          // inlet: state index; CL_PUB: publish slot
+    typename ST::R* inl_state;     // [n_inlet][27]
+    const NodeDev* inl;            // [n_inlet]
+    typename ST::R* pub_mac;       // [n_pub][2]: rho, u_x
+    typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
+    const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
+    double* partials;              // [ntiles][4] when diag != 0
+    Geo g;
+    Consts<typename ST::R> K;
+    typename ST::R rest[4];        // w*rho0 per weight class (deviation storage)
+    typename ST::R omega0, tau, cles;
+    int crev20;
+    int first;
+    int diag;
+    int closure;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; renumber so that
+// consecutive tiles (neighbouring x segments, which share partial cache
+// lines on odd steps) land on the same XCD and its L2.
+__device__ __forceinline__ int xcd_tile(int bid, int G)
+{
+    const int k = G >> 3, r = G & 7;
+    const int x = bid & 7, y = bid >> 3;
+    return (x < r) ? x * (k + 1) + y : r * (k + 1) + (x - r) * k + y;
+}
+
+struct Cell {
+    int x, y, z;
+    uint32_t i;
+    int dxm, dxp, dym, dyp, dzm, dzp;
+    bool inside;
+};
+
+__device__ __forceinline__ void set_deltas(const Geo& g, Cell& c)
+{
+    const int sx = 1, sy = g.nx, sz = g.nx * g.ny;
+    c.dxm = (c.x > 0 || !g.wx) ? -sx : (g.nx - 1) * sx;
+    c.dxp = (c.x < g.nx - 1 || !g.wx) ? sx : -(g.nx - 1) * sx;
+    c.dym = (c.y > 0 || !g.wy) ? -sy : (g.ny - 1) * sy;
+    c.dyp = (c.y < g.ny - 1 || !g.wy) ? sy : -(g.ny - 1) * sy;
+    c.dzm = (c.z > 0 || !g.wz) ? -sz : (g.nz - 1) * sz;
+    c.dzp = (c.z < g.nz - 1 || !g.wz) ? sz : -(g.nz - 1) * sz;
+}
+
+__device__ __forceinline__ Cell cell_at(const Geo& g, int x, int y, int z)
+{
+    Cell c;
+    c.x = x;
+    c.y = y;
+    c.z = z;
+    c.inside = true;
+    c.i = (uint32_t)(x + g.nx * (y + g.ny * z));
+    set_deltas(g, c);
+    return c;
+}
+
+__device__ __forceinline__ Cell cell_of(const Geo& g)
+{
+    Cell c;
+    const int t = xcd_tile((int)blockIdx.x, g.ntiles);
+    const int tx = t % g.ntx;
+    const int rem = t / g.ntx;
+    const int ty = rem % g.nty;
+    c.z = rem / g.nty;
+    c.x = (tx << g.bx_log2) + ((int)threadIdx.x & ((1 << g.bx_log2) - 1));
+    c.y = ty * g.by + ((int)threadIdx.x >> g.bx_log2);
+    c.inside = c.x < g.nx && c.y < g.ny;
+    c.i = (uint32_t)(c.x + g.nx * (c.y + g.ny * c.z));
+    set_deltas(g, c);
+    return c;
+}
+
+// Index of x + c_q.
+__device__ __forceinline__ uint32_t nb(const Cell& c, int q)
+{
+    int d = 0;
+    d += CX[q] > 0 ? c.dxp : (CX[q] < 0 ? c.dxm : 0);
+    d += CY[q] > 0 ? c.dyp : (CY[q] < 0 ? c.dym : 0);
+    d += CZ[q] > 0 ? c.dzp : (CZ[q] < 0 ? c.dzm : 0);
+    return (uint32_t)((int)c.i + d);
+}
+
+// Slot holding the pre-collision population q of cell c when the next step
+// has parity odd (the read location of the A-A scheme).
+__device__ __forceinline__ void read_slot(const Cell& c, int q, bool odd, int& plane, uint32_t& idx)
+{
+    if (odd) {
+        plane = OPP[q];
+        idx = nb(c, OPP[q]);
+    } else {
+        plane = q;
+        idx = c.i;
+    }
+}
+
+template <class ST, bool ODD>
+__device__ __forceinline__ void load_pop(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
+                                         typename ST::R* f)
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (ODD)
+            f[q] = ST::load(p[OPP[q]], nb(c, OPP[q]), rest[WCLASS[q]]);
+        else
+            f[q] = ST::load(p[q], c.i, rest[WCLASS[q]]);
+    }
+}
+
+// Streaming write of post-collision population q (goes to x + c_q).
+template <class ST, bool ODD>
+__device__ __forceinline__ void store_out(typename ST::T* const* p, const typename ST::R* rest, const Cell& c, int q,
+                                          typename ST::R v)
+{
+    if (ODD)
+        ST::store(p[q], nb(c, q), v, rest[WCLASS[q]]);
+    else
+        ST::store(p[OPP[q]], c.i, v, rest[WCLASS[q]]);
+}
+
+// Write the value cell x will read as incoming population q next step
+// (used when x - c_q is solid and nobody else writes that slot).
+template <class ST, bool ODD>
+__device__ __forceinline__ void store_in_next(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
+                                              int q, typename ST::R v)
+{
+    if (ODD)
+        ST::store(p[q], c.i, v, rest[WCLASS[q]]);
+    else
+        ST::store(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]]);
+}
+
+// Moving-wall update of an inlet cell's own state
+// (InletVelocityMovingWall.h:43-49): for n.c_q > 0,
+// f(x, c_rev[q]) = f*(x, q) - 6 w_q rho (c_q . u_w).  The dot products are
+// float as in the reference (its normals and velocities are float vectors).
+template <class R>
+__device__ __forceinline__ void moving_wall(const Consts<R>& K, const NodeDev& b, bool crev20, const R* post, R rho,
+                                            R* state)
+{
+    LBM_FP_STRICT
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const float nc = b.nx * (float)CX[q] + b.ny * (float)CY[q] + b.nz * (float)CZ[q];
+        if (nc > 0.0f) {
+            const float cu = (float)CX[q] * b.ux + (float)CY[q] * b.uy + (float)CZ[q] * b.uz;
+            const R v = post[q] - R(6) * K.w[WCLASS[q]] * rho * (R)cu;
+            if (q == 20) {
+                if (crev20) state[21] = v; else state[19] = v;
+            } else {
+                state[OPP[q]] = v;
+            }
+        }
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R fast_rcp(R x)
+{
+    return R(1) / x;
+}
+
+template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS>
+__global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
+{
+    using R = typename ST::R;
+    const Cell c = cell_of(a.g);
+    bool active = c.inside;
+    uint8_t cls = CL_FLUID;
+    if (FLAGS && active) cls = a.cls[c.i];
+    if (cls == CL_SOLID) active = false;
+    double dm[4] = {0.0, 0.0, 0.0, 0.0};
+
+    if (active) {
+        R f[Q];
+        int bi = -1;
+        const bool inlet = FLAGS && cls == CL_INLET;
+        if (FLAGS && cls >= CL_INLET) bi = a.bcidx[c.i];
+        if (inlet) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
+        } else {
+            load_pop<ST, ODD>(a.p, a.rest, c, f);
+        }
+
+        R rho, ux, uy, uz, omega;
+        R P[Q];
+        if (LIT) {
+            if (a.first) {
+                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+                rho = a.macro[c.i];
+                ux = a.macro[N + c.i];
+                uy = a.macro[2 * N + c.i];
+                uz = a.macro[3 * N + c.i];
+            } else {
+                moments_literal(f, rho, ux, uy, uz);
+            }
+            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
+                                                         : a.omega0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) P[q] = f[q];
+            if (COLL == LBM_COLLISION_CUM2015)
+                cum2015_literal(a.K, P, rho, ux, uy, uz, omega);
+            else
+                srt_literal(a.K, P, rho, ux, uy, uz, omega);
+            if (a.diag) {
+                dm[0] = (double)rho;
+                dm[1] = (double)(rho * ux);
+                dm[2] = (double)(rho * uy);
+                dm[3] = (double)(rho * uz);
+            }
+        } else {
+            const Raw2<R> r = raw_moments(f);
+            if (a.first) {
+                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+                rho = a.macro[c.i];
+                ux = a.macro[N + c.i];
+                uy = a.macro[2 * N + c.i];
+                uz = a.macro[3 * N + c.i];
+            } else {
+                rho = r.m0;
+                const R ir = fast_rcp(rho);
+                ux = r.jx * ir;
+                uy = r.jy * ir;
+                uz = r.jz * ir;
+            }
+            const Cen2<R> ce = central2(r, ux, uy, uz);
+            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+            if (COLL == LBM_COLLISION_CUM2015) {
+                cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) P[q] = f[q];
+                srt_fast(a.K, P, rho, ux, uy, uz, omega);
+            }
+            if (a.diag) {
+                dm[0] = (double)r.m0;
+                dm[1] = (double)r.jx;
+                dm[2] = (double)r.jy;
+                dm[3] = (double)r.jz;
+            }
+        }
+
+        if (!FLAGS || cls == CL_FLUID) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+        } else {
+            const uint64_t nw = a.node[c.i];
+            const uint32_t S = (uint32_t)(nw & MASK27);
+            const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
+            const bool wall = (nw >> 63) != 0;
+            if (inlet) {
+                // stream the pre-collision state (StreamingAB.h:62-65)
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
+                if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
+                    R st[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) st[q] = f[q];
+                    moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    if ((S >> q) & 1u) {
+                        R v;
+                        if (!wall)
+                            v = f[q];
+                        else if (q == 20)
+                            v = a.crev20 ? P[21] : P[19];
+                        else
+                            v = P[OPP[q]];
+                        store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
+                    }
+                }
+                // data the boundary patch kernels of this step read
+                if (cls == CL_PUB) {
+                    a.pub_mac[(size_t)bi * 2] = rho;
+                    a.pub_mac[(size_t)bi * 2 + 1] = ux;
+                    R* dst = a.pub_post + (size_t)bi * Q;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) dst[q] = P[q];
+                }
+            }
+        }
+    }
+
+    if (a.diag) {
+        // wave-level FP64 reduction, then across the 4 waves of the block
+        __shared__ double red[TPB / 64][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double v = dm[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            dm[k] = v;
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) red[wv][k] = dm[k];
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
+            a.partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary patch kernels (one thread per list entry), run after k_step on the
+// state the next step will read.
+// ---------------------------------------------------------------------------
+
+template <class ST>
+struct PatchArgs {
+    typename ST::T* p[Q];
+    const uint8_t* cls;
+    const int32_t* bcidx;
+    const NodeDev* nodes;          // this launch's entries
+    int n;
+    typename ST::R* inl_state;     // indexed like the inlet list
+    const typename ST::R* pub_mac;
+    const typename ST::R* pub_post;
+    Geo g;
+    Consts<typename ST::R> K;
+    typename ST::R rest[4];
+    typename ST::R nu, cs;
+    int crev20;
+    int odd;   // parity of the step that will read the state
+};
+
+// True when padded coordinate (x, y, z) lies on the caller's lattice
+// (halo cells do not; the reference skips neighbours outside its arrays).
+__device__ __forceinline__ bool on_lattice(const Geo& g, int x, int y, int z)
+{
+    return x >= g.px && x < g.px + g.ux && y >= g.py && y < g.py + g.uy && z >= g.pz && z < g.pz + g.uz;
+}
+
+__device__ __forceinline__ int wrap(int v, int n, int w) { return w ? (v + n) % n : v; }
+
+template <class ST>
+__device__ __forceinline__ void load_node(const PatchArgs<ST>& a, const Cell& c, typename ST::R* d, int* pl,
+                                          uint32_t* ix)
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        read_slot(c, q, a.odd != 0, pl[q], ix[q]);
+        d[q] = ST::load(a.p[pl[q]], ix[q], a.rest[WCLASS[q]]);
+    }
+}
+
+template <class ST>
+__device__ __forceinline__ void store_node(const PatchArgs<ST>& a, const typename ST::R* d, const int* pl,
+                                           const uint32_t* ix)
+{
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ST::store(a.p[pl[q]], ix[q], d[q], a.rest[WCLASS[q]]);
+}
+
+// InletVelocityEquilibrium::inlet (InletVelocityEquilibrium.h:51-103):
+// rho_b is extrapolated from rho and u_x of the first two interior
+// neighbours as they entered this step's collision, and the inlet state
+// becomes f_eq(u_b, rho_b).  Only +-x normals act, as in the reference.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
+{
+    LBM_FP_STRICT
+    using R = typename ST::R;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= a.n) return;
+    const NodeDev b = a.nodes[j];
+    if (b.kind != LBM_BC_INLET_EQUILIBRIUM) return;
+    const R uxb = b.ux, uyb = b.uy, uzb = b.uz;
+    const R rho1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * 2] : R(0);
+    const R ux1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * 2 + 1] : R(0);
+    const R ux2 = b.s1 >= 0 ? a.pub_mac[(size_t)b.s1 * 2 + 1] : R(0);
+    R rhob;
+    if (b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f) {
+        const R duxb = (uxb - ux1);
+        const R dux1 = (ux1 - ux2);
+        rhob = rho1 + a.nu / 3 * (dux1 - duxb);
+    } else if (b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f) {
+        const R duxb = (ux1 - uxb);
+        const R dux1 = (ux2 - ux1);
+        rhob = rho1 + a.nu / 3 * (duxb - dux1);
+    } else {
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.inl_state[(size_t)j * Q + q] = feq_q(a.K, q, uxb, uyb, uzb, rhob);
+}
+
+// OutletDensityInterpolated::outlet (OutletDensityInterpolated.h:170-252):
+// irregular +x nodes first copy f_crev into f_q where the cell at
+// (x, y - c_y, z - c_z) is solid (sequentially in q), then incoming
+// populations are interpolated between the node's and the upstream cell's
+// post-collision values and the density is reset to the outlet density.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_outlet(PatchArgs<ST> a)
+{
+    LBM_FP_STRICT
+    using R = typename ST::R;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= a.n) return;
+    const NodeDev b = a.nodes[j];
+    const bool xp = b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f;
+    const bool xm = b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f;
+    if (!xp && !xm) return;
+    const Cell c = cell_at(a.g, b.x, b.y, b.z);
+    R d[Q];
+    int pl[Q];
+    uint32_t ix[Q];
+    load_node(a, c, d, pl, ix);
+    if (xp && !b.regular) {
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel) {
+            const int yy = wrap(b.y - CY[vel], a.g.ny, a.g.wy), zz = wrap(b.z - CZ[vel], a.g.nz, a.g.wz);
+            if (on_lattice(a.g, b.x, yy, zz) && a.cls[(uint32_t)(b.x + a.g.nx * (yy + a.g.ny * zz))] == CL_SOLID)
+                d[vel] = d[crev_of(vel, a.crev20 != 0)];
+        }
+    }
+    if (b.s1 != -2) {
+        const R* self = a.pub_post + (size_t)b.s0 * Q;
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel) {
+            const float nc = b.nx * (float)CX[vel] + b.ny * (float)CY[vel] + b.nz * (float)CZ[vel];
+            if (nc < 0.0f) {
+                const R up = b.s1 >= 0 ? a.pub_post[(size_t)b.s1 * Q + vel] : R(0);
+                d[vel] = a.cs * up + (R(1) - a.cs) * self[vel];
+            }
+        }
+        const R density = b.rho;
+        const R rho = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8] + d[9] + d[10] + d[11] + d[12] +
+                      d[13] + d[14] + d[15] + d[16] + d[17] + d[18] + d[19] + d[20] + d[21] + d[22] + d[23] + d[24] +
+                      d[25] + d[26];
+        const R ux = (d[1] - d[2] + d[7] - d[8] + d[9] - d[10] - d[11] + d[12] - d[15] + d[16] - d[19] + d[20] -
+                      d[21] + d[22] + d[23] - d[24] - d[25] + d[26]) / density;
+        const R uy = (-d[5] + d[6] - d[11] + d[12] + d[13] - d[14] + d[15] - d[16] + d[17] - d[18] + d[19] -
+                      d[20] - d[21] + d[22] - d[23] + d[24] - d[25] + d[26]) / density;
+        const R uz = (-d[3] + d[4] - d[7] + d[8] + d[9] - d[10] - d[13] + d[14] + d[17] - d[18] - d[19] + d[20] +
+                      d[21] - d[22] - d[23] + d[24] - d[25] + d[26]) / density;
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel)
+            d[vel] = d[vel] - feq_q(a.K, vel, ux, uy, uz, rho) + feq_q(a.K, vel, ux, uy, uz, R(1));
+    }
+    store_node(a, d, pl, ix);
+}
+
+// Periodic::periodic (Periodic.h:43-138): populations with n.c < 0 are
+// copied from the partner cell (the node with its normal-axis index replaced
+// by the periodic index); irregular nodes also bounce their own
+// post-collision populations back off solid neighbours, sequentially in q.
+// Entries that depend on each other run in separate launches, in list order.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_periodic(PatchArgs<ST> a)
+{
+    using R = typename ST::R;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= a.n) return;
+    const NodeDev b = a.nodes[j];
+    const Cell c = cell_at(a.g, b.x, b.y, b.z);
+    R d[Q];
+    int pl[Q];
+    uint32_t ix[Q];
+    load_node(a, c, d, pl, ix);
+    const uint32_t pi = (uint32_t)b.partner;
+    const bool self_partner = pi == c.i;
+    const uint8_t pcls = a.cls[pi];
+    const int px = (int)(pi % (uint32_t)a.g.nx);
+    const int py = (int)((pi / (uint32_t)a.g.nx) % (uint32_t)a.g.ny);
+    const int pz = (int)(pi / ((uint32_t)a.g.nx * (uint32_t)a.g.ny));
+    const Cell pc = cell_at(a.g, px, py, pz);
+    const R* post = a.pub_post + (size_t)b.s0 * Q;
+#pragma unroll
+    for (int vel = 0; vel < Q; ++vel) {
+        const float nc = b.nx * (float)CX[vel] + b.ny * (float)CY[vel] + b.nz * (float)CZ[vel];
+        if (nc < 0.0f && !self_partner) {
+            R v;
+            if (pcls == CL_SOLID) {
+                v = R(0);
+            } else if (pcls == CL_INLET) {
+                v = a.inl_state[(size_t)a.bcidx[pi] * Q + vel];
+            } else {
+                int ppl;
+                uint32_t pix;
+                read_slot(pc, vel, a.odd != 0, ppl, pix);
+                v = ST::load(a.p[ppl], pix, a.rest[WCLASS[vel]]);
+            }
+            d[vel] = v;
+        }
+        if (!b.regular && nc <= 0.0f) {
+            const int xx = wrap(b.x + CX[vel], a.g.nx, a.g.wx);
+            const int yy = wrap(b.y + CY[vel], a.g.ny, a.g.wy);
+            const int zz = wrap(b.z + CZ[vel], a.g.nz, a.g.wz);
+            if (on_lattice(a.g, xx, yy, zz) && a.cls[(uint32_t)(xx + a.g.nx * (yy + a.g.ny * zz))] == CL_SOLID)
+                d[crev_of(vel, a.crev20 != 0)] = post[vel];
+        }
+    }
+    store_node(a, d, pl, ix);
+}
+
+// ---------------------------------------------------------------------------
+// Auxiliary kernels: initialisation, gather/scatter, moments, omega, sums.
+// These iterate over the caller's cells (no halo); io is in caller layout.
+// ---------------------------------------------------------------------------
+
+template <class ST>
+struct AuxArgs {
+    typename ST::T* p[Q];
+    const uint8_t* cls;
+    const int32_t* bcidx;
+    typename ST::R* inl_state;
+    typename ST::R* macro;   // rho, ux, uy, uz planes (padded layout)
+    Geo g;
+    Consts<typename ST::R> K;
+    typename ST::R rest[4];
+    typename ST::R tau, cles, omega0;
+    int closure;
+    int lit;
+    int odd;        // parity of the stored state
+    int first;      // no step taken since lbm_init_equilibrium: rho, u are the macro planes
+    int q;          // plane for gather/scatter
+    double* io;     // staging buffer (FP64, caller layout)
+};
+
+// Thread -> caller cell -> padded cell.
+__device__ __forceinline__ bool user_cell(const Geo& g, uint32_t& iu, Cell& c)
+{
+    const size_t NU = (size_t)g.ux * g.uy * g.uz;
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= NU) return false;
+    iu = (uint32_t)i;
+    const int x = (int)(iu % (uint32_t)g.ux);
+    const uint32_t r = iu / (uint32_t)g.ux;
+    const int y = (int)(r % (uint32_t)g.uy);
+    const int z = (int)(r / (uint32_t)g.uy);
+    c = cell_at(g, x + g.px, y + g.py, z + g.pz);
+    return true;
+}
+
+// Current populations of a non-solid cell.
+template <class ST>
+__device__ __forceinline__ void cell_state(const AuxArgs<ST>& a, const Cell& c, uint8_t cls, typename ST::R* f)
+{
+    if (cls == CL_INLET) {
+        const int bi = a.bcidx[c.i];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
+    } else if (a.odd) {
+        load_pop<ST, true>(a.p, a.rest, c, f);
+    } else {
+        load_pop<ST, false>(a.p, a.rest, c, f);
+    }
+}
+
+// f = f_eq(rho, u) from the macro planes (InitializationEquilibriumConstVector.h:111-126);
+// runs over the padded lattice.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_init(AuxArgs<ST> a)
+{
+    using R = typename ST::R;
+    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    const size_t ii = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (ii >= N) return;
+    const uint32_t i = (uint32_t)ii;
+    const uint8_t cls = a.cls[i];
+    const R rho = a.macro[i], ux = a.macro[N + i], uy = a.macro[2 * N + i], uz = a.macro[3 * N + i];
+    const bool inlet = cls == CL_INLET;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const R v = cls == CL_SOLID ? R(0) : feq_q(a.K, q, ux, uy, uz, rho);
+        ST::store(a.p[q], i, v, a.rest[WCLASS[q]]);
+        if (inlet) a.inl_state[(size_t)a.bcidx[i] * Q + q] = v;
+    }
+}
+
+// Write plane q of a host-provided state (FP64 staging) at the current parity.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_scatter(AuxArgs<ST> a)
+{
+    using R = typename ST::R;
+    uint32_t iu;
+    Cell c;
+    if (!user_cell(a.g, iu, c)) return;
+    const uint8_t cls = a.cls[c.i];
+    if (cls == CL_SOLID) return;
+    const int q = a.q;
+    const R v = (R)a.io[iu];
+    if (cls == CL_INLET) {
+        a.inl_state[(size_t)a.bcidx[c.i] * Q + q] = v;
+        return;
+    }
+    int pl;
+    uint32_t ix;
+    read_slot(c, q, a.odd != 0, pl, ix);
+    ST::store(a.p[pl], ix, v, a.rest[WCLASS[q]]);
+}
+
+// Gather plane q of the current state into FP64 staging (0 on solids).
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_gather(AuxArgs<ST> a)
+{
+    uint32_t iu;
+    Cell c;
+    if (!user_cell(a.g, iu, c)) return;
+    const uint8_t cls = a.cls[c.i];
+    if (cls == CL_SOLID) {
+        a.io[iu] = 0.0;
+        return;
+    }
+    const int q = a.q;
+    if (cls == CL_INLET) {
+        a.io[iu] = (double)a.inl_state[(size_t)a.bcidx[c.i] * Q + q];
+        return;
+    }
+    int pl;
+    uint32_t ix;
+    read_slot(c, q, a.odd != 0, pl, ix);
+    a.io[iu] = (double)ST::load(a.p[pl], ix, a.rest[WCLASS[q]]);
+}
+
+// rho, u and omega of the current state into 5 FP64 staging planes.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
+{
+    using R = typename ST::R;
+    uint32_t iu;
+    Cell c;
+    if (!user_cell(a.g, iu, c)) return;
+    const size_t NU = (size_t)a.g.ux * a.g.uy * a.g.uz;
+    const uint8_t cls = a.cls[c.i];
+    if (cls == CL_SOLID) {
+        for (int k = 0; k < 5; ++k) a.io[k * NU + iu] = 0.0;
+        return;
+    }
+    R f[Q];
+    cell_state(a, c, cls, f);
+    R rho, ux, uy, uz, om;
+    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    if (a.lit) {
+        if (a.first) {
+            rho = a.macro[c.i];
+            ux = a.macro[N + c.i];
+            uy = a.macro[2 * N + c.i];
+            uz = a.macro[3 * N + c.i];
+        } else {
+            moments_literal(f, rho, ux, uy, uz);
+        }
+        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
+                                                  : a.omega0;
+    } else {
+        const Raw2<R> r = raw_moments(f);
+        if (a.first) {
+            rho = a.macro[c.i];
+            ux = a.macro[N + c.i];
+            uy = a.macro[2 * N + c.i];
+            uz = a.macro[3 * N + c.i];
+        } else {
+            rho = r.m0;
+            ux = r.jx / rho;
+            uy = r.jy / rho;
+            uz = r.jz / rho;
+        }
+        const Cen2<R> ce = central2(r, ux, uy, uz);
+        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+    }
+    a.io[iu] = (double)rho;
+    a.io[NU + iu] = (double)ux;
+    a.io[2 * NU + iu] = (double)uy;
+    a.io[3 * NU + iu] = (double)uz;
+    a.io[4 * NU + iu] = (double)om;
+}
+
+// rho, u of a single cell (probe), caller coordinates.
+template <class ST>
+__global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
+{
+    using R = typename ST::R;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Cell c = cell_at(a.g, x + a.g.px, y + a.g.py, z + a.g.pz);
+    const uint8_t cls = a.cls[c.i];
+    if (cls == CL_SOLID) {
+        for (int k = 0; k < 4; ++k) a.io[k] = 0.0;
+        return;
+    }
+    R rho, ux, uy, uz;
+    if (a.first) {
+        const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+        rho = a.macro[c.i];
+        ux = a.macro[N + c.i];
+        uy = a.macro[2 * N + c.i];
+        uz = a.macro[3 * N + c.i];
+    } else {
+        R f[Q];
+        cell_state(a, c, cls, f);
+        moments_literal(f, rho, ux, uy, uz);
+    }
+    a.io[0] = (double)rho;
+    a.io[1] = (double)ux;
+    a.io[2] = (double)uy;
+    a.io[3] = (double)uz;
+}
+
+// Per-block FP64 sums of mass and momentum over non-solid cells.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_sums(AuxArgs<ST> a)
+{
+    using R = typename ST::R;
+    uint32_t iu;
+    Cell c;
+    double dm[4] = {0.0, 0.0, 0.0, 0.0};
+    if (user_cell(a.g, iu, c) && a.cls[c.i] != CL_SOLID) {
+        R f[Q];
+        cell_state(a, c, a.cls[c.i], f);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double v = (double)f[q];
+            dm[0] += v;
+            dm[1] += CX[q] * v;
+            dm[2] += CY[q] * v;
+            dm[3] += CZ[q] * v;
+        }
+    }
+    __shared__ double red[TPB / 64][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double v = dm[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        dm[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) red[wv][k] = dm[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
+        a.io[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+// Plain stream copy used as the bandwidth reference.
+__global__ __launch_bounds__(TPB) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+}  // namespace lbm

@@ -128,3 +128,71 @@ def cavity(n: int, lid_velocity=(0.05, 0.0, 0.0)):
     bc = [dict(kind=BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls]
     bc += [dict(kind=BC_INLET_MOVING_WALL, **d) for d in inl]
     return flags, bc
+
+
+def channel(nx, ny, nz, u_in=0.04, rho_out=1.0, ghost_z=True, obstacle=True, inlet_kind=BC_INLET_EQUILIBRIUM,
+            walls=True):
+    """Channel along x: solid rows at y = 0 and y = ny - 1, an equilibrium
+    velocity inlet on x = 0 (outward normal -x), an interpolated density
+    outlet on x = nx - 1 (+x) and, with ghost_z, ghost-layer periodicity
+    along z (z = 0 copies from z = nz - 2, z = nz - 1 from z = 1, the pattern
+    of the reference's main.cu:178-179).  Cells sit at i + 0.5 (additional 0).
+
+    walls=False drops the solid rows (the caller then wraps y).
+    Meshing order: wall, periodic, inlet, outlet, compile -- so the edge
+    cells (x = nx - 1, z = 0 / nz - 1) are both outlet and ghost nodes and
+    the ghost nodes at x = 0 have inlet cells as partners.
+    Returns (flags, bc) with bc a list of dicts."""
+    g = np.ones((nz, ny, nx), np.int32)
+    if walls:
+        g[:, 0, :] = 0
+        g[:, ny - 1, :] = 0
+    if obstacle:
+        x0, y0 = nx // 3, ny // 2 - 1
+        g[2:nz - 2, y0:y0 + 2, x0:x0 + 2] = 0
+    flags, walls = meshing_boundary_wall(g)
+    kw = dict(additional=0.0)
+    shape = flags.shape
+    bc = []
+    per = []
+    if ghost_z:
+        m0 = cuboid_mask(shape, (-1.0, -1.0, 0.2), (nx + 1.0, ny + 1.0, 0.8), (-1.0, ny + 1.0, 0.8), **kw)
+        per += meshing_periodic(flags, m0, -3, (0.0, 0.0, -1.0), nz - 2)
+        m1 = cuboid_mask(shape, (-1.0, -1.0, nz - 0.8), (nx + 1.0, ny + 1.0, nz - 0.2), (-1.0, ny + 1.0, nz - 0.2),
+                         **kw)
+        per += meshing_periodic(flags, m1, -3, (0.0, 0.0, 1.0), 1)
+    mi = cuboid_mask(shape, (0.2, -1.0, -1.0), (0.8, ny + 1.0, nz + 1.0), (0.8, ny + 1.0, -1.0),
+                     lookup_dtype=np.float32, **kw)
+    vel = (float(np.float32(u_in)), 0.0, 0.0)
+    inl = meshing_inlet_uniform(flags, mi, -1, (-1.0, 0.0, 0.0), vel)
+    mo = cuboid_mask(shape, (nx - 0.8, -1.0, -1.0), (nx - 0.2, ny + 1.0, nz + 1.0), (nx - 0.2, ny + 1.0, -1.0), **kw)
+    out = meshing_outlet(flags, mo, -2, (1.0, 0.0, 0.0), rho_out)
+    walls = compile_wall(flags, walls)
+    bc += [dict(kind=BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls]
+    bc += [dict(kind=inlet_kind, **d) for d in inl]
+    bc += [dict(kind=BC_OUTLET_INTERPOLATED, **d) for d in out]
+    bc += [dict(kind=BC_PERIODIC_GHOST, **d) for d in per]
+    return flags, bc
+
+
+def ghost_box(m):
+    """m^3 fluid cells with ghost-layer periodicity on all six faces
+    (x faces first, then y, then z; face cuboids include edges and corners,
+    so those cells appear in two or three periodic entries)."""
+    g = np.ones((m, m, m), np.int32)
+    flags, _ = meshing_boundary_wall(g)
+    shape = flags.shape
+    per = []
+    big = m + 1.0
+    for axis in range(3):
+        for side, (lo, hi, idx) in enumerate(((0.2, 0.8, m - 2), (m - 0.8, m - 0.2, 1))):
+            p1 = [-1.0, -1.0, -1.0]
+            p2 = [big, big, big]
+            p1[axis], p2[axis] = lo, hi
+            p3 = list(p2)
+            p3[0] = p1[0]
+            nrm = [0.0, 0.0, 0.0]
+            nrm[axis] = -1.0 if side == 0 else 1.0
+            mask = cuboid_mask(shape, tuple(p1), tuple(p2), tuple(p3), additional=0.0)
+            per += meshing_periodic(flags, mask, -3, tuple(nrm), idx)
+    return flags, [dict(kind=BC_PERIODIC_GHOST, **d) for d in per]

@@ -46,12 +46,23 @@ def test_validation_rejects_bad_descriptors():
     flags, bc, _ = cavity_setup(4)
     # tau <= 1/2 (omega >= 2)
     assert _create_status(make_desc(flags, bc, tau=0.5)) == abi.LBM_E_ARG
-    # fluid cell on a non-periodic array boundary (no halo)
-    assert _create_status(make_desc(np.ones((4, 4, 4), np.int32), [], tau=0.6)) == abi.LBM_E_ARG
-    # inlet flag without an inlet entry
+    # wall node whose neighbour lies outside the lattice on a non-periodic axis
+    # (the reference reads outside its arrays there)
+    ones = np.ones((4, 4, 4), np.int32)
+    assert _create_status(make_desc(ones, [abi.BcNode(0, 1, 1, abi.BC_WALL)], tau=0.6)) == abi.LBM_E_UNSUPPORTED
+    # two inlet entries on one cell
     f2 = flags.copy()
     f2[2, 2, 2] = -1
-    assert _create_status(make_desc(f2, bc, tau=0.6)) == abi.LBM_E_ARG
+    inl = abi.BcNode(2, 2, 2, abi.BC_INLET_MOVING_WALL)
+    assert _create_status(make_desc(f2, list(bc) + [inl, inl], tau=0.6)) == abi.LBM_E_ARG
+    # outlet entry on an inlet cell
+    out = abi.BcNode(2, 2, 2, abi.BC_OUTLET_INTERPOLATED, nx=1.0, rho=1.0)
+    assert _create_status(make_desc(f2, list(bc) + [inl, out], tau=0.6)) == abi.LBM_E_UNSUPPORTED
+    # periodic partner index outside the lattice
+    f4 = flags.copy()
+    f4[1, 1, 1] = -3
+    gh = abi.BcNode(1, 1, 1, abi.BC_PERIODIC_GHOST, nx=-1.0, per_idx=4)
+    assert _create_status(make_desc(f4, list(bc) + [gh], tau=0.6)) == abi.LBM_E_ARG
     # unknown negative id
     f3 = flags.copy()
     f3[2, 2, 2] = -7
@@ -70,5 +81,10 @@ def test_valid_descriptor_without_device_reports_hip_error():
         pytest.skip("a GPU is visible")
     flags, bc, _ = cavity_setup(4)
     assert _create_status(make_desc(flags, bc, tau=0.6)) == abi.LBM_E_HIP
+    # fluid on a non-periodic array boundary and a flag -1 cell without an
+    # inlet entry are valid (the core pads a solid halo; the cell keeps its state)
+    ones = np.ones((4, 4, 4), np.int32)
+    ones[1, 1, 1] = -1
+    assert _create_status(make_desc(ones, [], tau=0.6)) == abi.LBM_E_HIP
     with pytest.raises(LbmError):
         Lattice(make_desc(flags, bc, tau=0.6))

@@ -1,0 +1,144 @@
+"""Equilibrium inlet, interpolated outlet and ghost-layer periodic rules of
+the HIP core vs the CPU oracle, through the C ABI.
+
+The channel geometry (oracle.mesher_oracle.channel) has wall nodes, an
+obstacle, an equilibrium inlet on the array boundary x = 0, an outlet on
+x = nx - 1 and ghost layers along z; its edge cells are both outlet and
+ghost nodes and some ghost partners are inlet cells.  The ghost box puts
+edge and corner cells into two or three periodic entries, whose list order
+the core keeps by launching dependent entries separately.
+FP64 literal must match the oracle bit for bit (bar <= 1e-12 relative L2).
+"""
+import numpy as np
+import pytest
+
+from helpers import OracleLattice, bc_nodes, random_velocity_field, rel_l2
+from lb_amd import Lattice, abi, make_desc
+from oracle import mesher_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.01
+TAU = 3 * NU + 0.5
+
+
+def _channel(nx=24, ny=12, nz=10, **kw):
+    flags, bc = mo.channel(nx, ny, nz, **kw)
+    return flags, bc_nodes(bc)
+
+
+def _init(flags, seed=5, u0=0.04):
+    rng = np.random.default_rng(seed)
+    solid = flags == 0
+    ux = np.where(solid, 0.0, u0 + 0.005 * rng.standard_normal(flags.shape))
+    uy = np.where(solid, 0.0, 0.005 * rng.standard_normal(flags.shape))
+    rho = np.where(solid, 1.0, 1.0 + 0.002 * rng.standard_normal(flags.shape))
+    return rho, ux, uy, None
+
+
+def _run_pair(flags, bc, steps, init, oracle_storage=abi.STORAGE_F64, chunks=None, **kw):
+    g = Lattice(make_desc(flags, bc, **kw))
+    o = OracleLattice(make_desc(flags, bc, **{**kw, "storage": oracle_storage}))
+    g.init_equilibrium(*init)
+    o.init_equilibrium(*init)
+    for n in (chunks or [steps]):
+        g.step(n)
+    o.step(steps)
+    g.sync()
+    return g, o
+
+
+def _errors(tag, g, o, flags):
+    fluid = flags != 0
+    fg, fo = g.populations(), o.populations()
+    mg, mo_ = g.macroscopic(), o.macroscopic()
+    ef = rel_l2(fg[:, fluid], fo[:, fluid])
+    er = rel_l2(mg[0][fluid], mo_[0][fluid])
+    eu = rel_l2(np.stack([m[fluid] for m in mg[1:]]), np.stack([m[fluid] for m in mo_[1:]]))
+    print(f"[{tag}] relL2 f={ef:.3e} rho={er:.3e} u={eu:.3e} max|df|={np.abs(fg - fo).max():.3e}", flush=True)
+    return ef, er, eu
+
+
+@pytest.mark.parametrize("compat", [abi.COMPAT_REFERENCE, 0], ids=["reference", "corrected"])
+def test_channel_fp64_literal_cumulant_les(gpu, compat):
+    flags, bc = _channel()
+    g, o = _run_pair(flags, bc, 150, _init(flags), collision=abi.COLLISION_CUM2015,
+                     closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=abi.STORAGE_F64,
+                     arith=abi.ARITH_LITERAL, compat=compat, tau=TAU, nu=NU)
+    ef, er, eu = _errors(f"channel fp64 lit cum+les compat={compat}", g, o, flags)
+    assert ef <= 1e-12 and er <= 1e-12 and eu <= 1e-12
+    assert np.array_equal(g.omega(), o.omega())
+
+
+def test_channel_fp64_literal_srt_moving_wall_inlet(gpu):
+    """Same channel with the moving-wall inlet (no patch kernel) and BGK."""
+    flags, bc = _channel(inlet_kind=mo.BC_INLET_MOVING_WALL)
+    g, o = _run_pair(flags, bc, 120, _init(flags, seed=6), collision=abi.COLLISION_SRT,
+                     storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    ef, er, eu = _errors("channel fp64 lit srt mw", g, o, flags)
+    assert ef <= 1e-12 and eu <= 1e-12
+
+
+def test_channel_fp32_literal_vs_fp32_oracle(gpu):
+    flags, bc = _channel()
+    g, o = _run_pair(flags, bc, 100, _init(flags, seed=7), oracle_storage=abi.STORAGE_F32,
+                     collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17,
+                     storage=abi.STORAGE_F32, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    fg, fo = g.populations(), o.populations()
+    print(f"[channel fp32 lit] max|df|={np.abs(fg - fo).max():.3e}", flush=True)
+    assert np.array_equal(fg, fo)
+
+
+def test_channel_chunked_steps_match(gpu):
+    flags, bc = _channel(nx=16, ny=8, nz=6)
+    kw = dict(collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+              compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    a = Lattice(make_desc(flags, bc, **kw))
+    b = Lattice(make_desc(flags, bc, **kw))
+    init = _init(flags, seed=8)
+    a.init_equilibrium(*init)
+    b.init_equilibrium(*init)
+    a.step(23)
+    for n in (1, 2, 7, 13):
+        b.step(n)
+    assert np.array_equal(a.populations(), b.populations())
+
+
+# FP64 fast: the Smagorinsky rate takes sqrt(|Pi_neq|), and |Pi_neq| of a
+# near-equilibrium start is rounding noise (~1e-17), so the two operation
+# orders differ by ~1e-9 there; the bound is set an order above that.
+@pytest.mark.parametrize("storage,tol", [(abi.STORAGE_F64, 1e-8), (abi.STORAGE_F32, 2e-5),
+                                         (abi.STORAGE_F16DEV, 2e-3), (abi.STORAGE_BF16DEV, 1e-2)],
+                         ids=["f64", "f32", "f16dev", "bf16dev"])
+def test_channel_fast_vs_fp64_oracle(gpu, storage, tol):
+    flags, bc = _channel()
+    g, o = _run_pair(flags, bc, 100, _init(flags, seed=9), collision=abi.COLLISION_CUM2015,
+                     closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=storage, arith=abi.ARITH_FAST,
+                     compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    ef, er, eu = _errors(f"channel fast storage={storage}", g, o, flags)
+    assert eu <= tol and er <= tol
+
+
+def test_ghost_box_six_faces_fp64_literal(gpu):
+    flags, bcd = mo.ghost_box(10)
+    bc = bc_nodes(bcd)
+    u = random_velocity_field(flags.shape, 0.03, seed=11)
+    g, o = _run_pair(flags, bc, 80, (None, u[0], u[1], u[2]), collision=abi.COLLISION_CUM2015,
+                     closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=abi.STORAGE_F64,
+                     arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=0.52)
+    ef, er, eu = _errors("ghost box 10^3 fp64 lit", g, o, flags)
+    assert ef <= 1e-12 and eu <= 1e-12
+
+
+def test_uniform_flow_channel_steady_on_gpu(gpu):
+    """Uniform u0 at rho = 1 through inlet, outlet and ghost layers stays an
+    equilibrium to rounding (a fixed point of every rule)."""
+    flags, bc = _channel(nx=20, ny=6, nz=6, obstacle=False, walls=False)
+    u0 = float(np.float32(0.04))
+    g = Lattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                          storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    g.init_equilibrium(None, np.full(flags.shape, u0), None, None)
+    g.step(40)
+    rho, ux, uy, uz = g.macroscopic()
+    assert np.abs(rho - 1).max() < 1e-14 and np.abs(ux - u0).max() < 1e-14
+    assert np.abs(uy).max() < 1e-14 and np.abs(uz).max() < 1e-14

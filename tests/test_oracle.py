@@ -306,3 +306,109 @@ def test_product_collision_arithmetic_is_bitwise_the_oracle():
             wo = O.orc_cell_omega_les(abi.COMPAT_REFERENCE, dbl, abi.ptr(f), *m, 0.5015, 0.173)
             wh = H.host_omega_les(0, dbl, abi.COMPAT_REFERENCE, abi.ptr(f), *m, 0.5015, 0.173)
             assert wo == wh
+
+
+# --- inlet / outlet / ghost-layer rules -----------------------------------
+
+def _channel(nx, ny, nz, **kw):
+    from helpers import bc_nodes
+    from oracle import mesher_oracle as mo
+
+    flags, bc = mo.channel(nx, ny, nz, **kw)
+    return flags, bc, bc_nodes(bc)
+
+
+def test_ghost_layer_with_walls_equals_index_wrap_for_z_invariant_flow():
+    """Ghost layers beside walls: irregular ghost nodes bounce their own
+    post-collision populations back off solids (Periodic.h:92-133).  For a
+    z-invariant flow in a y-walled channel this must equal index wrap along z
+    exactly (true opposite table: the ghost rule DF(crev v) = DP(v) and the
+    wall rule DF(q) = DP(crev q) coincide only when crev is an involution)."""
+    from helpers import bc_nodes
+    from oracle import mesher_oracle as mo
+
+    nx, ny, nzi = 9, 8, 4
+    g = np.ones((nzi + 2, ny, nx), np.int32)
+    g[:, 0, :] = 0
+    g[:, ny - 1, :] = 0
+    flags_g, walls_g = mo.meshing_boundary_wall(g)
+    sh = flags_g.shape
+    m0 = mo.cuboid_mask(sh, (-50, -50, 0.2), (50, 50, 0.8), (-50, 50, 0.8), additional=0.0)
+    m1 = mo.cuboid_mask(sh, (-50, -50, nzi + 1.2), (50, 50, nzi + 1.8), (-50, 50, nzi + 1.8), additional=0.0)
+    per = mo.meshing_periodic(flags_g, m0, -3, (0.0, 0.0, -1.0), nzi)
+    per += mo.meshing_periodic(flags_g, m1, -3, (0.0, 0.0, 1.0), 1)
+    assert any(p["regular"] == 0 and p["y"] == 1 for p in per)
+    walls_g = mo.compile_wall(flags_g, walls_g)
+    bc_g = bc_nodes([dict(kind=mo.BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls_g] +
+                    [dict(kind=mo.BC_PERIODIC_GHOST, **p) for p in per])
+    gw = g[1:-1]
+    flags_w, walls_w = mo.meshing_boundary_wall(gw)
+    bc_w = bc_nodes([dict(kind=mo.BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls_w])
+    y = np.arange(ny)[None, :, None]
+    x = np.arange(nx)[None, None, :]
+    ux = np.broadcast_to(0.03 * np.sin(np.pi * y / ny) + 0.01 * np.cos(2 * np.pi * x / nx), gw.shape)
+    uy = np.broadcast_to(0.01 * np.sin(2 * np.pi * x / nx) * (y > 0) * (y < ny - 1), gw.shape)
+    kw = dict(collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17,
+              storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.56)
+    ow = OracleLattice(make_desc(flags_w, bc_w, periodic=(1, 0, 1), **kw))
+    og = OracleLattice(make_desc(flags_g, bc_g, periodic=(1, 0, 0), **kw))
+    solid_w = flags_w == 0
+    ow.init_equilibrium(None, np.where(solid_w, 0, ux), np.where(solid_w, 0, uy), None)
+    pad = lambda a: np.concatenate([a[-1:], a, a[:1]], axis=0)  # noqa: E731
+    og.init_equilibrium(None, pad(np.where(solid_w, 0, ux)), pad(np.where(solid_w, 0, uy)), None)
+    ow.step(30)
+    og.step(30)
+    fw, fg = ow.populations(), og.populations()
+    assert np.abs(fw).max() > 0 and np.array_equal(fw, fg[:, 1:-1])
+    assert np.array_equal(fg[:, 0], fw[:, -1]) and np.array_equal(fg[:, -1], fw[:, 0])
+
+
+def test_uniform_flow_through_inlet_outlet_channel_is_steady():
+    """A uniform equilibrium flow u0 at rho = 1 is a fixed point of the
+    equilibrium inlet (rho_b extrapolates to 1, state = f_eq(u0, 1)), the
+    interpolated outlet (interpolating equal values, density reset to 1) and
+    the ghost layers (copying equal values)."""
+    u0 = 0.04
+    flags, _, bc = _channel(12, 6, 6, u_in=u0, obstacle=False, walls=False)
+    assert (flags == -1).sum() and (flags == -2).sum() and (flags == -3).sum()
+    o = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    u = np.float64(np.float32(u0))
+    o.init_equilibrium(None, np.full(flags.shape, u), None, None)
+    o.step(20)
+    f = o.populations()
+    expect = feq(1.0, (u, 0.0, 0.0))
+    assert np.abs(f - expect[:, None, None, None]).max() < 1e-15
+
+
+def test_outlet_resets_density_and_inlet_holds_equilibrium():
+    """After every step the outlet nodes hold total mass 1 exactly (exact
+    weights: f - f_eq(u, rho) + f_eq(u, 1) with sum f_eq(u, r) = r,
+    OutletDensityInterpolated.h:244-246), and every inlet node holds
+    f_eq(u_b, rho_b) with rho_b = rho(x+1) - (nu/3)(u_b - 2 u(x+1) + u(x+2))
+    from the neighbours' moments entering the step (InletVelocityEquilibrium.h:80-92,
+    written here as a second difference)."""
+    nu = 0.02
+    flags, bcd, bc = _channel(16, 8, 6, u_in=0.05)
+    o = OracleLattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+                                c_les=0.17, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+                                compat=0, tau=3 * nu + 0.5, nu=nu))
+    rng = np.random.default_rng(3)
+    solid = flags == 0
+    o.init_equilibrium(1 + 0.01 * rng.standard_normal(flags.shape), np.where(solid, 0, 0.03), None, None)
+    outlets = [d for d in bcd if d["kind"] == 3 and flags[d["z"], d["y"], d["x"]] == -2 and 0 < d["z"] < 5]
+    inlets = [d for d in bcd if d["kind"] == 2]
+    assert outlets and inlets
+    for _ in range(4):
+        rho0, ux0, _, _ = o.macroscopic()
+        o.step(1)
+        f = o.populations()
+        for d in outlets:
+            assert abs(f[:, d["z"], d["y"], d["x"]].sum() - 1.0) < 1e-14
+        for d in inlets:
+            x, y, z = d["x"], d["y"], d["z"]
+            ub = np.float64(np.float32(d["velocity"][0]))
+            rb = rho0[z, y, x + 1] - nu / 3 * (ub - 2 * ux0[z, y, x + 1] + ux0[z, y, x + 2])
+            got = f[:, z, y, x]
+            assert abs(got.sum() - rb) < 1e-13
+            assert np.abs(got - feq(rb, (ub, 0.0, 0.0))).max() < 1e-15
