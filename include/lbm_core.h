@@ -105,9 +105,14 @@ typedef struct lbm_desc {
     const lbm_bc_node* bc;     /* host, copied; may be NULL when n_bc == 0 */
 } lbm_desc;
 
-/* Create a context on the current HIP device.  Validates the flag grid: every
- * non-solid cell must have all 26 neighbours inside the array or across an
- * index-wrap axis (the reference has no bounds checks, StreamingAB.h:51-56). */
+/* Create a context on the current HIP device.  Validates the flag grid and the
+ * boundary lists.  Neighbours outside the array on an axis without index wrap
+ * are treated as the reference's out-of-array reads are restated: populations
+ * streaming from there are left untouched, streaming there is dropped (the
+ * core pads a one-cell solid halo for their slots; StreamingAB.h:51-56 has no
+ * bounds checks).  A wall-list node with such a neighbour is
+ * LBM_E_UNSUPPORTED, as are flags below -3, outlet or ghost entries on inlet
+ * cells, and inlet/outlet rules that would read an inlet cell as a neighbour. */
 lbm_status lbm_create(const lbm_desc* desc, lbm_ctx** out);
 
 /* Equilibrium initialisation f = f_eq(rho, u) on every non-solid cell
@@ -155,8 +160,10 @@ lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
 /* Steps completed so far. */
 lbm_status lbm_step_count(lbm_ctx* ctx, int64_t* steps);
 
-/* Average device time of the collide-stream kernel over the last lbm_step
- * call, in milliseconds (HIP events on the context stream). */
+/* Average device time of one step over the last lbm_step call, in
+ * milliseconds (HIP events on the context stream): the collide-stream kernel
+ * plus any boundary patch kernels (equilibrium inlet, interpolated outlet,
+ * ghost-layer periodic). */
 lbm_status lbm_last_kernel_ms(lbm_ctx* ctx, double* ms);
 
 /* Device-to-device copy of `bytes` bytes (a plain float4 stream kernel) run
