@@ -72,6 +72,29 @@ CORE_SYMBOLS = [
     ("lbm_destroy", None, [_vp]),
 ]
 
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+MLP_FEATURES = 12
+MLP_PARAMS = 5057
+
+# (name, restype, argtypes) of every entry point declared in include/lbm_closure.h
+CLOSURE_SYMBOLS = [
+    ("lbm_mlp_create", ctypes.c_int, [_fp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("lbm_mlp_set_data", ctypes.c_int, [_vp, _fp, _fp, _fp, ctypes.c_int64]),
+    ("lbm_mlp_gradient", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _dp,
+                                        _fp]),
+    ("lbm_mlp_train", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_uint64, _dp]),
+    ("lbm_mlp_forward", ctypes.c_int, [_vp, _fp, ctypes.c_int64, ctypes.c_float, _fp]),
+    ("lbm_mlp_get_params", ctypes.c_int, [_vp, _fp]),
+    ("lbm_mlp_set_params", ctypes.c_int, [_vp, _fp]),
+    ("lbm_mlp_destroy", None, [_vp]),
+    ("lbm_set_closure", ctypes.c_int, [_vp, _fp, _fp]),
+    ("lbm_get_closure_features", ctypes.c_int, [_vp, _fp, _ip, _dp]),
+    ("lbm_filter", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+]
+
 _core = None
 
 
@@ -82,7 +105,7 @@ def load_core() -> ctypes.CDLL:
         if not os.path.exists(CORE_LIB):
             raise RuntimeError(f"HIP core {CORE_LIB} is not built; run __graft_entry__.build()")
         lib = ctypes.CDLL(CORE_LIB)
-        for name, res, args in CORE_SYMBOLS:
+        for name, res, args in CORE_SYMBOLS + CLOSURE_SYMBOLS:
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args

@@ -125,3 +125,32 @@ class Lattice:
         v = ctypes.c_size_t()
         _check(self._lib.lbm_device_bytes(self._h, ctypes.byref(v)), "lbm_device_bytes")
         return v.value
+
+    # -- learned closure and filtering -------------------------------------
+    def set_closure(self, params: np.ndarray, norm: np.ndarray):
+        """Install closure weights (NPARAM floats) and normalisation
+        {mu, sd, lo, hi} (4 x 12 floats); lattice created with CLOSURE_MLP."""
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        nm = np.ascontiguousarray(norm, dtype=np.float32).ravel()
+        if p.size != abi.MLP_PARAMS or nm.size != 4 * abi.MLP_FEATURES:
+            raise ValueError("set_closure: wrong parameter or normalisation size")
+        _check(self._lib.lbm_set_closure(self._h, abi.ptr(p, ctypes.c_float), abi.ptr(nm, ctypes.c_float)),
+               "lbm_set_closure")
+
+    def closure_features(self):
+        """(normalised features [12, nz, ny, nx], envelope flag, coefficient)."""
+        f = np.empty((abi.MLP_FEATURES, *self.shape), np.float32)
+        e = np.empty(self.shape, np.int32)
+        c = np.empty(self.shape)
+        _check(self._lib.lbm_get_closure_features(self._h, abi.ptr(f, ctypes.c_float), abi.ptr(e, ctypes.c_int32),
+                                                  abi.ptr(c)), "lbm_get_closure_features")
+        return f, e.astype(bool), c
+
+    def filter(self, factor: int):
+        """Box filter over factor^3 blocks: 10 arrays (rho, ux, uy, uz, uxux,
+        uyuy, uzuz, uxuy, uxuz, uyuz) shaped (nz/f, ny/f, nx/f)."""
+        nz, ny, nx = self.shape
+        cs = (nz // factor, ny // factor, nx // factor)
+        out = np.empty((10, *cs))
+        _check(self._lib.lbm_filter(self._h, int(factor), abi.ptr(out)), "lbm_filter")
+        return out

@@ -380,6 +380,7 @@ struct Cen2 {
 template <class R>
 LBM_HD Cen2<R> central2(const Raw2<R>& r, R ux, R uy, R uz)
 {
+    LBM_FP_STRICT  // same bits in every kernel that forms the Smagorinsky rate
     Cen2<R> c;
     c.k100 = r.jx - ux * r.m0;
     c.k010 = r.jy - uy * r.m0;
@@ -400,6 +401,9 @@ LBM_HD Cen2<R> central2(const Raw2<R>& r, R ux, R uy, R uz)
 template <class R>
 LBM_HD R omega_les_fast(const Cen2<R>& c, R rho, R tau, R cles)
 {
+    // contraction off: the learned closure's fallback evaluates this same
+    // formula in another kernel and must give the same bits
+    LBM_FP_STRICT
     const R third = rho * R(1.0 / 3.0);
     const R pxx = c.kxx - third, pyy = c.kyy - third, pzz = c.kzz - third;
     const R n2 = pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (c.kxy * c.kxy + c.kxz * c.kxz + c.kyz * c.kyz);

@@ -17,9 +17,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/lbm_closure.h"
 #include "lbm_kernels.hpp"
 
 using namespace lbm;
+
+// closure.hip: BF16 inference image of a parameter vector
+void lbm_mlp_pack_bf16(const float* p, uint16_t* w1, uint16_t* w2);
 
 struct lbm_ctx {
     lbm_desc d;
@@ -54,6 +58,12 @@ struct lbm_ctx {
     bool nan_seen = false;
     double last_ms = 0.0;
     size_t dev_bytes = 0;
+    // learned closure
+    bool mlp_ready = false;
+    float* cl_params = nullptr;   // FP32 parameters
+    uint16_t* cl_wb = nullptr;    // BF16 W1 [64][16] + W2 [64][64]
+    float cl_b3 = 0.0f;
+    MlpNorm cl_norm{};
 };
 
 #define HIPCHK(x)                                   \
@@ -103,6 +113,19 @@ static void fill_common(const lbm_ctx* c, typename ST::T** p, typename ST::R* re
     for (int k = 0; k < 4; ++k) rest[k] = ST::dev ? (R)(K.w[k] * (R)c->d.rho0) : R(0);
     typename ST::T* base = (typename ST::T*)c->pops;
     for (int q = 0; q < Q; ++q) p[q] = base + (size_t)q * c->N;
+}
+
+static MlpWeights closure_weights(const lbm_ctx* c)
+{
+    MlpWeights W;
+    W.w1 = c->cl_wb;
+    W.w2 = c->cl_wb ? c->cl_wb + NHID * 16 : nullptr;
+    W.b1 = c->cl_params ? c->cl_params + OFF_B1 : nullptr;
+    W.b2 = c->cl_params ? c->cl_params + OFF_B2 : nullptr;
+    W.w3 = c->cl_params ? c->cl_params + OFF_W3 : nullptr;
+    W.b3 = c->cl_b3;
+    W.c = (float)c->d.c_les;
+    return W;
 }
 
 template <class ST>
@@ -199,6 +222,13 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.cles = (R)c->d.c_les;
     a.crev20 = (c->d.compat & LBM_COMPAT_CREV20) ? 1 : 0;
     a.closure = c->d.closure;
+    const bool mlp = c->d.closure == LBM_CLOSURE_MLP;
+    MlpStepArgs<ST> ma;
+    if (mlp) {
+        if (!c->mlp_ready) return LBM_E_STATE;
+        ma.mlp = closure_weights(c);
+        ma.norm = c->cl_norm;
+    }
     const bool patches = c->n_eq || c->n_outlet || c->n_ghost;
     const dim3 grid((unsigned)c->g.ntiles), block(TPB);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -206,12 +236,26 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         a.first = c->first ? 1 : 0;
         a.diag = 0;
         const bool odd = (c->steps & 1) != 0;
-        if (c->flags_needed) {
-            if (odd) k_step<ST, COLL, LIT, true, true><<<grid, block, 0, c->stream>>>(a);
-            else k_step<ST, COLL, LIT, false, true><<<grid, block, 0, c->stream>>>(a);
-        } else {
-            if (odd) k_step<ST, COLL, LIT, true, false><<<grid, block, 0, c->stream>>>(a);
-            else k_step<ST, COLL, LIT, false, false><<<grid, block, 0, c->stream>>>(a);
+        if constexpr (sizeof(R) == 4 && !LIT) {
+            if (mlp) {
+                ma.s = a;
+                if (c->flags_needed) {
+                    if (odd) k_step_mlp<ST, COLL, true, true><<<grid, block, 0, c->stream>>>(ma);
+                    else k_step_mlp<ST, COLL, false, true><<<grid, block, 0, c->stream>>>(ma);
+                } else {
+                    if (odd) k_step_mlp<ST, COLL, true, false><<<grid, block, 0, c->stream>>>(ma);
+                    else k_step_mlp<ST, COLL, false, false><<<grid, block, 0, c->stream>>>(ma);
+                }
+            }
+        }
+        if (!mlp) {
+            if (c->flags_needed) {
+                if (odd) k_step<ST, COLL, LIT, true, true><<<grid, block, 0, c->stream>>>(a);
+                else k_step<ST, COLL, LIT, false, true><<<grid, block, 0, c->stream>>>(a);
+            } else {
+                if (odd) k_step<ST, COLL, LIT, true, false><<<grid, block, 0, c->stream>>>(a);
+                else k_step<ST, COLL, LIT, false, false><<<grid, block, 0, c->stream>>>(a);
+            }
         }
         HIPCHK(hipGetLastError());
         if (patches) {
@@ -594,8 +638,9 @@ const char* lbm_status_str(lbm_status s)
 void lbm_destroy(lbm_ctx* c)
 {
     if (!c) return;
-    void* bufs[] = {c->pops,  c->cls,     c->node,     c->bcidx, c->inl_state, c->inl,     c->outl,
-                    c->ghost, c->pub_mac, c->pub_post, c->macro, c->stage,     c->partials};
+    void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
+                    c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
+                    c->cl_wb};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -611,14 +656,17 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !d->flags) return LBM_E_ARG;
     if (d->n_bc < 0 || (d->n_bc > 0 && !d->bc)) return LBM_E_ARG;
     if (d->collision != LBM_COLLISION_SRT && d->collision != LBM_COLLISION_CUM2015) return LBM_E_ARG;
-    if (d->closure == LBM_CLOSURE_MLP) return LBM_E_UNSUPPORTED;
-    if (d->closure != LBM_CLOSURE_NONE && d->closure != LBM_CLOSURE_SMAGORINSKY) return LBM_E_ARG;
+    if (d->closure != LBM_CLOSURE_NONE && d->closure != LBM_CLOSURE_SMAGORINSKY && d->closure != LBM_CLOSURE_MLP)
+        return LBM_E_ARG;
+    // the learned closure runs in the fast FP32 arithmetic
+    if (d->closure == LBM_CLOSURE_MLP && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
+    if (d->closure == LBM_CLOSURE_MLP && d->storage == LBM_STORAGE_F64) return LBM_E_UNSUPPORTED;
     if (d->storage < LBM_STORAGE_F64 || d->storage > LBM_STORAGE_BF16DEV) return LBM_E_ARG;
     if (d->arith != LBM_ARITH_LITERAL && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->arith == LBM_ARITH_LITERAL && d->storage >= LBM_STORAGE_F16DEV) return LBM_E_ARG;
     // tau > 1/2 keeps omega in (0, 2); NonDim...h:79 asserts this (compiled out by -DNDEBUG)
     if (!(d->tau > 0.5) || !std::isfinite(d->tau)) return LBM_E_ARG;
-    if (d->closure == LBM_CLOSURE_SMAGORINSKY && !(d->c_les >= 0.0)) return LBM_E_ARG;
+    if (d->closure != LBM_CLOSURE_NONE && !(d->c_les >= 0.0)) return LBM_E_ARG;
     if (d->periodic[0] && d->nx < 2) return LBM_E_ARG;
     if (d->periodic[1] && d->ny < 2) return LBM_E_ARG;
     if (d->periodic[2] && d->nz < 2) return LBM_E_ARG;
@@ -826,6 +874,30 @@ static lbm_status run_macro(lbm_ctx* c)
     });
 }
 
+// Closure read-back into device buffers (any may be null).
+static lbm_status run_closure_eval(lbm_ctx* c, float* feats, int32_t* env, double* coef, double* omega)
+{
+    return with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        if constexpr (sizeof(typename ST::R) != 4) {
+            return LBM_E_UNSUPPORTED;
+        } else {
+            ClosureEvalArgs<ST> e;
+            std::memset(&e, 0, sizeof(e));
+            e.a = aux_args<ST>(c);
+            e.mlp = closure_weights(c);
+            e.norm = c->cl_norm;
+            e.feats = feats;
+            e.env = env;
+            e.coef = coef;
+            e.omega = omega;
+            k_closure_eval<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(e);
+            HIPCHK(hipGetLastError());
+            return LBM_OK;
+        }
+    });
+}
+
 lbm_status lbm_get_macroscopic(lbm_ctx* c, double* rho, double* ux, double* uy, double* uz)
 {
     if (!c) return LBM_E_ARG;
@@ -869,7 +941,12 @@ lbm_status lbm_get_omega(lbm_ctx* c, double* omega)
     if (!c->initialized) return LBM_E_STATE;
     lbm_status st = lbm_sync(c);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    st = run_macro(c);
+    if (c->d.closure == LBM_CLOSURE_MLP) {
+        if (!c->mlp_ready) return LBM_E_STATE;
+        st = run_closure_eval(c, nullptr, nullptr, nullptr, c->stage + 4 * c->NU);
+    } else {
+        st = run_macro(c);
+    }
     if (st != LBM_OK) return st;
     HIPCHK(hipMemcpyAsync(omega, c->stage + 4 * c->NU, c->NU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1007,6 +1084,93 @@ lbm_status lbm_device_bytes(lbm_ctx* c, size_t* bytes)
 {
     if (!c || !bytes) return LBM_E_ARG;
     *bytes = c->dev_bytes;
+    return LBM_OK;
+}
+
+lbm_status lbm_set_closure(lbm_ctx* c, const float* params, const float* norm)
+{
+    if (!c || !params || !norm) return LBM_E_ARG;
+    if (c->d.closure != LBM_CLOSURE_MLP) return LBM_E_STATE;
+    MlpNorm N;
+    for (int k = 0; k < NFEAT; ++k) {
+        N.mu[k] = norm[k];
+        N.sd[k] = norm[NFEAT + k];
+        N.lo[k] = norm[2 * NFEAT + k];
+        N.hi[k] = norm[3 * NFEAT + k];
+        if (!(N.sd[k] > 0.0f) || !std::isfinite(N.sd[k]) || !std::isfinite(N.mu[k])) return LBM_E_ARG;
+    }
+    for (int i = 0; i < NPARAM; ++i)
+        if (!std::isfinite(params[i])) return LBM_E_ARG;
+    if (!c->cl_params) {
+        lbm_status st = dev_alloc(c, (void**)&c->cl_params, sizeof(float) * NPARAM);
+        if (st != LBM_OK) return st;
+    }
+    if (!c->cl_wb) {
+        lbm_status st = dev_alloc(c, (void**)&c->cl_wb, sizeof(uint16_t) * (NHID * 16 + NHID * NHID));
+        if (st != LBM_OK) return st;
+    }
+    std::vector<uint16_t> wb(NHID * 16 + NHID * NHID);
+    lbm_mlp_pack_bf16(params, wb.data(), wb.data() + NHID * 16);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->cl_params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->cl_wb, wb.data(), sizeof(uint16_t) * wb.size(), hipMemcpyHostToDevice));
+    c->cl_b3 = params[OFF_B3];
+    c->cl_norm = N;
+    c->mlp_ready = true;
+    return LBM_OK;
+}
+
+lbm_status lbm_get_closure_features(lbm_ctx* c, float* features, int32_t* in_envelope, double* coefficient)
+{
+    if (!c) return LBM_E_ARG;
+    if (c->d.closure != LBM_CLOSURE_MLP || !c->mlp_ready || !c->initialized) return LBM_E_STATE;
+    lbm_status st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    const size_t NU = c->NU;
+    float* df = nullptr;
+    int32_t* de = nullptr;
+    double* dc = nullptr;
+    st = LBM_OK;
+    if (features && hipMalloc(&df, sizeof(float) * NFEAT * NU) != hipSuccess) st = LBM_E_OOM;
+    if (st == LBM_OK && in_envelope && hipMalloc(&de, sizeof(int32_t) * NU) != hipSuccess) st = LBM_E_OOM;
+    if (st == LBM_OK && coefficient) dc = c->stage;
+    if (st == LBM_OK) st = run_closure_eval(c, df, de, dc, nullptr);
+    if (st == LBM_OK && df &&
+        hipMemcpyAsync(features, df, sizeof(float) * NFEAT * NU, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        st = LBM_E_HIP;
+    if (st == LBM_OK && de &&
+        hipMemcpyAsync(in_envelope, de, sizeof(int32_t) * NU, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        st = LBM_E_HIP;
+    if (st == LBM_OK && dc &&
+        hipMemcpyAsync(coefficient, dc, sizeof(double) * NU, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        st = LBM_E_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess && st == LBM_OK) st = LBM_E_HIP;
+    (void)hipGetLastError();
+    if (df) (void)hipFree(df);
+    if (de) (void)hipFree(de);
+    return st;
+}
+
+lbm_status lbm_filter(lbm_ctx* c, int32_t factor, double* out)
+{
+    if (!c || !out || factor < 2) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    const int fx = c->g.ux, fy = c->g.uy, fz = c->g.uz;
+    if (fx % factor || fy % factor || fz % factor) return LBM_E_ARG;
+    lbm_status st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    const int cx = fx / factor, cy = fy / factor, cz = fz / factor;
+    const size_t NC = (size_t)cx * cy * cz;
+    st = with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        k_filter<ST><<<blocks_for(NC), TPB, 0, c->stream>>>(a, factor, cx, cy, cz);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (st != LBM_OK) return st;
+    HIPCHK(hipMemcpyAsync(out, c->stage, 10 * NC * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
 

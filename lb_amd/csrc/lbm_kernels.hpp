@@ -39,6 +39,7 @@
 
 #include "../../include/lbm_core.h"
 #include "collide.hpp"
+#include "mlp.hpp"
 #include "storage.hpp"
 
 namespace lbm {
@@ -100,6 +101,14 @@ struct StepArgs {
     int first;
     int diag;
     int closure;
+};
+
+// Arguments of the collide-stream step with the learned closure.
+template <class ST>
+struct MlpStepArgs {
+    StepArgs<ST> s;
+    MlpWeights mlp;
+    MlpNorm norm;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber so that
@@ -247,6 +256,66 @@ __device__ __forceinline__ R fast_rcp(R x)
     return R(1) / x;
 }
 
+// Streaming, bounce-back and boundary bookkeeping of one cell after its
+// collision: f = pre-collision populations, P = post-collision populations.
+// (k_step keeps its own inline copy of this code: passing the population
+// arrays through a helper raised its register count from 48-64 to > 100.)
+template <class ST, bool ODD, bool FLAGS>
+__device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c, uint8_t cls, int bi, bool inlet,
+                                            const typename ST::R (&f)[Q], const typename ST::R (&P)[Q],
+                                            typename ST::R rho, typename ST::R ux)
+{
+    using R = typename ST::R;
+    if (!FLAGS || cls == CL_FLUID) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+    } else {
+        const uint64_t nw = a.node[c.i];
+        const uint32_t S = (uint32_t)(nw & MASK27);
+        const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
+        const bool wall = (nw >> 63) != 0;
+        if (inlet) {
+            // stream the pre-collision state (StreamingAB.h:62-65)
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
+            if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
+                R st[Q];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) st[q] = f[q];
+                moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if ((S >> q) & 1u) {
+                    R v;
+                    if (!wall)
+                        v = f[q];
+                    else if (q == 20)
+                        v = a.crev20 ? P[21] : P[19];
+                    else
+                        v = P[OPP[q]];
+                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
+                }
+            }
+            // data the boundary patch kernels of this step read
+            if (cls == CL_PUB) {
+                a.pub_mac[(size_t)bi * 2] = rho;
+                a.pub_mac[(size_t)bi * 2 + 1] = ux;
+                R* dst = a.pub_post + (size_t)bi * Q;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) dst[q] = P[q];
+            }
+        }
+    }
+}
+
 template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS>
 __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 {
@@ -329,7 +398,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         }
 
         if (!FLAGS || cls == CL_FLUID) {
-#pragma unroll
+    #pragma unroll
             for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
         } else {
             const uint64_t nw = a.node[c.i];
@@ -338,22 +407,22 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             const bool wall = (nw >> 63) != 0;
             if (inlet) {
                 // stream the pre-collision state (StreamingAB.h:62-65)
-#pragma unroll
+    #pragma unroll
                 for (int q = 0; q < Q; ++q)
                     if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
                 if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                     R st[Q];
-#pragma unroll
+    #pragma unroll
                     for (int q = 0; q < Q; ++q) st[q] = f[q];
                     moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
-#pragma unroll
+    #pragma unroll
                     for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
                 }
             } else {
-#pragma unroll
+    #pragma unroll
                 for (int q = 0; q < Q; ++q)
                     if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
-#pragma unroll
+    #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     if ((S >> q) & 1u) {
                         R v;
@@ -371,7 +440,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                     a.pub_mac[(size_t)bi * 2] = rho;
                     a.pub_mac[(size_t)bi * 2 + 1] = ux;
                     R* dst = a.pub_post + (size_t)bi * Q;
-#pragma unroll
+    #pragma unroll
                     for (int q = 0; q < Q; ++q) dst[q] = P[q];
                 }
             }
@@ -400,6 +469,102 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
             a.partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
         }
+    }
+}
+
+// Moments, Smagorinsky rate and closure features of one cell, shared by the
+// collide-stream kernel and the closure read-back kernel.
+template <class ST>
+struct ClosurePrep {
+    typename ST::R rho, ux, uy, uz, m0, pin, om_s;
+    Cen2<typename ST::R> ce;
+    float x[NFEAT];   // raw features
+    float xn[NFEAT];  // normalised features
+    bool env;         // inside the training envelope
+};
+
+template <class ST>
+__device__ __forceinline__ void closure_prep(const typename ST::R* f, bool first, const typename ST::R* macro,
+                                             size_t N, uint32_t i, typename ST::R tau, typename ST::R cles,
+                                             const MlpNorm& norm, ClosurePrep<ST>& o)
+{
+    using R = typename ST::R;
+    const Raw2<R> r = raw_moments(f);
+    o.m0 = r.m0;
+    if (first) {
+        o.rho = macro[i];
+        o.ux = macro[N + i];
+        o.uy = macro[2 * N + i];
+        o.uz = macro[3 * N + i];
+    } else {
+        o.rho = r.m0;
+        const R ir = fast_rcp(o.rho);
+        o.ux = r.jx * ir;
+        o.uy = r.jy * ir;
+        o.uz = r.jz * ir;
+    }
+    o.ce = central2(r, o.ux, o.uy, o.uz);
+    o.om_s = omega_les_fast(o.ce, o.rho, tau, cles);
+    R om_unused;
+    cell_features(o.ce, o.rho, tau, cles, o.x, o.pin, om_unused);
+    o.env = mlp_normalise(norm, o.x, o.xn);
+}
+
+// Collide-stream step with the learned closure (fast arithmetic, FP32 math).
+// Phase 1 loads a cell and forms its features; phase 2 evaluates the network
+// for the wave's 64 cells on the matrix cores (uniform control flow: every
+// lane takes part, cells without a collision feed zeros); phase 3 collides
+// with omega from the network's coefficient -- or from the Smagorinsky
+// formula when a feature leaves the training envelope -- and streams.
+template <class ST, int COLL, bool ODD, bool FLAGS>
+__global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
+{
+    const StepArgs<ST>& a = ma.s;
+    using R = typename ST::R;
+    static_assert(sizeof(R) == 4, "the learned closure runs in FP32 arithmetic");
+    __shared__ MlpLds L;
+    mlp_load_lds(L, ma.mlp);
+    __syncthreads();
+
+    const Cell c = cell_of(a.g);
+    bool active = c.inside;
+    uint8_t cls = CL_FLUID;
+    if (FLAGS && active) cls = a.cls[c.i];
+    if (cls == CL_SOLID) active = false;
+    const bool inlet = FLAGS && cls == CL_INLET;
+    int bi = -1;
+    R f[Q];
+    ClosurePrep<ST> cp;
+    float xn[NFEAT];
+#pragma unroll
+    for (int k = 0; k < NFEAT; ++k) xn[k] = 0.0f;
+    cp.env = false;
+    if (active) {
+        if (FLAGS && cls >= CL_INLET) bi = a.bcidx[c.i];
+        if (inlet) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
+        } else {
+            load_pop<ST, ODD>(a.p, a.rest, c, f);
+        }
+        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, ma.norm, cp);
+        if (cp.env) {
+#pragma unroll
+            for (int k = 0; k < NFEAT; ++k) xn[k] = cp.xn[k];
+        }
+    }
+    const float z = mlp_wave_forward(L, xn, ma.mlp.b3);
+    if (active) {
+        const R omega = cp.env ? omega_from_coefficient(mlp_coefficient(z, ma.mlp.c), cp.pin, a.tau) : cp.om_s;
+        R P[Q];
+        if (COLL == LBM_COLLISION_CUM2015) {
+            cum2015_fast(a.K, cp.ce, cp.m0, cp.rho, cp.ux, cp.uy, cp.uz, omega, P);
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) P[q] = f[q];
+            srt_fast(a.K, P, cp.rho, cp.ux, cp.uy, cp.uz, omega);
+        }
+        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, f, P, cp.rho, cp.ux);
     }
 }
 
@@ -758,7 +923,15 @@ __global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
             uy = r.jy / rho;
             uz = r.jz / rho;
         }
-        const Cen2<R> ce = central2(r, ux, uy, uz);
+        // the rate from the velocity as the step forms it (reciprocal, then products)
+        R vx = ux, vy = uy, vz = uz;
+        if (!a.first) {
+            const R ir = fast_rcp(rho);
+            vx = r.jx * ir;
+            vy = r.jy * ir;
+            vz = r.jz * ir;
+        }
+        const Cen2<R> ce = central2(r, vx, vy, vz);
         om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
     }
     a.io[iu] = (double)rho;
@@ -835,6 +1008,108 @@ __global__ __launch_bounds__(TPB) void k_sums(AuxArgs<ST> a)
         for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
         a.io[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
     }
+}
+
+template <class ST>
+struct ClosureEvalArgs {
+    AuxArgs<ST> a;
+    MlpWeights mlp;
+    MlpNorm norm;
+    float* feats;     // [12][NU] normalised features, or null
+    int32_t* env;     // [NU], or null
+    double* coef;     // [NU] coefficient the next step uses, or null
+    double* omega;    // [NU] rate the next step uses, or null
+};
+
+// Closure inputs and outputs of the current state, per caller cell; the
+// same per-cell code as k_step_mlp.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_closure_eval(ClosureEvalArgs<ST> e)
+{
+    using R = typename ST::R;
+    __shared__ MlpLds L;
+    mlp_load_lds(L, e.mlp);
+    __syncthreads();
+    const AuxArgs<ST>& a = e.a;
+    uint32_t iu = 0;
+    Cell c;
+    const bool have = user_cell(a.g, iu, c);
+    const bool active = have && a.cls[c.i] != CL_SOLID;
+    ClosurePrep<ST> cp;
+    cp.env = false;
+    float xn[NFEAT];
+#pragma unroll
+    for (int k = 0; k < NFEAT; ++k) xn[k] = 0.0f;
+    if (active) {
+        R f[Q];
+        cell_state(a, c, a.cls[c.i], f);
+        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, e.norm,
+                         cp);
+        if (cp.env) {
+#pragma unroll
+            for (int k = 0; k < NFEAT; ++k) xn[k] = cp.xn[k];
+        }
+    }
+    const float z = mlp_wave_forward(L, xn, e.mlp.b3);
+    if (!have) return;
+    const size_t NU = (size_t)a.g.ux * a.g.uy * a.g.uz;
+    if (e.feats)
+        for (int k = 0; k < NFEAT; ++k) e.feats[k * NU + iu] = active ? cp.xn[k] : 0.0f;
+    if (e.env) e.env[iu] = (active && cp.env) ? 1 : 0;
+    const float C = cp.env ? mlp_coefficient(z, e.mlp.c) : (float)a.cles;
+    if (e.coef) e.coef[iu] = active ? (double)C : 0.0;
+    if (e.omega)
+        e.omega[iu] = active ? (double)(cp.env ? omega_from_coefficient(C, cp.pin, a.tau) : cp.om_s) : 0.0;
+}
+
+// Box filter over fct^3 blocks of caller cells: averages of rho, u and u u
+// over the non-solid cells of each block, 10 FP64 planes in a.io.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_filter(AuxArgs<ST> a, int fct, int cx, int cy, int cz)
+{
+    using R = typename ST::R;
+    const size_t NC = (size_t)cx * cy * cz;
+    const size_t ic = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (ic >= NC) return;
+    const int X = (int)(ic % (size_t)cx), Y = (int)((ic / (size_t)cx) % (size_t)cy), Z = (int)(ic / ((size_t)cx * cy));
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+    int cnt = 0;
+    for (int dz = 0; dz < fct; ++dz)
+        for (int dy = 0; dy < fct; ++dy)
+            for (int dx = 0; dx < fct; ++dx) {
+                const Cell c = cell_at(a.g, X * fct + dx + a.g.px, Y * fct + dy + a.g.py, Z * fct + dz + a.g.pz);
+                const uint8_t cls = a.cls[c.i];
+                if (cls == CL_SOLID) continue;
+                R rho, ux, uy, uz;
+                if (a.first) {
+                    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+                    rho = a.macro[c.i];
+                    ux = a.macro[N + c.i];
+                    uy = a.macro[2 * N + c.i];
+                    uz = a.macro[3 * N + c.i];
+                } else {
+                    R f[Q];
+                    cell_state(a, c, cls, f);
+                    moments_literal(f, rho, ux, uy, uz);
+                }
+                const double u = ux, v = uy, w = uz;
+                acc[0] += (double)rho;
+                acc[1] += u;
+                acc[2] += v;
+                acc[3] += w;
+                acc[4] += u * u;
+                acc[5] += v * v;
+                acc[6] += w * w;
+                acc[7] += u * v;
+                acc[8] += u * w;
+                acc[9] += v * w;
+                ++cnt;
+            }
+    const double inv = cnt ? 1.0 / cnt : 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) a.io[k * NC + ic] = acc[k] * inv;
 }
 
 // Plain stream copy used as the bandwidth reference.

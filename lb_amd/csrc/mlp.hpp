@@ -1,0 +1,177 @@
+// mlp.hpp -- the closure network on CDNA4 matrix cores (device code).
+//
+// Network: 12 -> 64 -> 64 -> 1, SiLU hidden activations, BF16 weights and
+// activations into v_mfma_f32_16x16x16_bf16, FP32 accumulation; output head
+// C = c_les * softplus(z) / ln 2 (z = 0 gives the Smagorinsky constant).
+//
+// Matrix-core layout (16x16x16, one wave): D[16x16] = A[16x16] B[16x16] + C;
+// lane l = 16 q + s holds A[s][4q+j], B[4q+j][s] and D[4q+j][s], j = 0..3.
+// Every layer is computed transposed -- hidden units along M, samples along
+// N -- so that a layer's D fragment is the next layer's B fragment as is:
+//   H1^T = W1 X^T,  H2^T = W2 A1^T,  z = w3 . A2 + b3.
+// A wave evaluates 64 samples as four tiles of 16; the 16 samples of tile t
+// are the cells of lanes 16t .. 16t+15, whose features are staged in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "features.hpp"
+
+namespace lbm {
+
+constexpr int NHID = 64;
+constexpr int OFF_W1 = 0;                      // [64][12]
+constexpr int OFF_B1 = OFF_W1 + NHID * NFEAT;  // [64]
+constexpr int OFF_W2 = OFF_B1 + NHID;          // [64][64]
+constexpr int OFF_B2 = OFF_W2 + NHID * NHID;   // [64]
+constexpr int OFF_W3 = OFF_B2 + NHID;          // [64]
+constexpr int OFF_B3 = OFF_W3 + NHID;          // [1]
+constexpr int NPARAM = OFF_B3 + 1;             // 5057
+constexpr float LN2F = 0.69314718055994531f;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bits, round to nearest even.
+__device__ __forceinline__ short f2bf(float x)
+{
+    uint32_t b = __float_as_uint(x);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (short)(b >> 16);
+}
+
+// Logistic function from the hardware exp2 and reciprocal.
+__device__ __forceinline__ float sigmoid_hw(float h)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * h));
+}
+
+__device__ __forceinline__ float silu_hw(float h) { return h * sigmoid_hw(h); }
+
+__device__ __forceinline__ float dsilu_hw(float h)
+{
+    const float s = sigmoid_hw(h);
+    return s * (1.0f + h * (1.0f - s));
+}
+
+__device__ __forceinline__ float softplus_f(float z) { return z > 20.0f ? z : log1pf(expf(z)); }
+
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Inference weights as the kernels read them.
+struct MlpWeights {
+    const uint16_t* w1;   // [64][16] BF16, columns 12..15 zero
+    const uint16_t* w2;   // [64][64] BF16
+    const float* b1;      // [64]
+    const float* b2;      // [64]
+    const float* w3;      // [64]
+    float b3;
+    float c;              // head scale (Smagorinsky constant)
+};
+
+// Input normalisation and training envelope (normalised units).
+struct MlpNorm {
+    float mu[NFEAT], sd[NFEAT], lo[NFEAT], hi[NFEAT];
+};
+
+// LDS image of the weights plus a per-wave feature staging area.
+struct MlpLds {
+    uint16_t w1[NHID * 16];
+    uint16_t w2[NHID * NHID];
+    float b1[NHID], b2[NHID], w3[NHID];
+    uint16_t xs[4][64 * 16];   // per wave: [lane][feature], BF16
+};
+
+// Cooperative copy of the weights into LDS by a 256-thread workgroup.
+// Call in uniform control flow, followed by a workgroup barrier.
+__device__ __forceinline__ void mlp_load_lds(MlpLds& L, const MlpWeights& W)
+{
+    const int t = threadIdx.x;
+    const uint32_t* s1 = (const uint32_t*)W.w1;
+    const uint32_t* s2 = (const uint32_t*)W.w2;
+    uint32_t* d1 = (uint32_t*)L.w1;
+    uint32_t* d2 = (uint32_t*)L.w2;
+    for (int i = t; i < NHID * 16 / 2; i += 256) d1[i] = s1[i];
+    for (int i = t; i < NHID * NHID / 2; i += 256) d2[i] = s2[i];
+    if (t < NHID) {
+        L.b1[t] = W.b1[t];
+        L.b2[t] = W.b2[t];
+        L.w3[t] = W.w3[t];
+    }
+}
+
+// Ordering of LDS accesses between lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// z of this lane's sample (normalised features xn).  Must be called by all
+// 64 lanes of the wave in uniform control flow; lanes without a sample pass
+// zeros and ignore the result.
+__device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, float b3)
+{
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
+    const int q = lane >> 4, s = lane & 15;
+    uint16_t* xs = L.xs[wave];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) xs[lane * 16 + k] = k < NFEAT ? (uint16_t)f2bf(xn[k]) : (uint16_t)0;
+    wave_lds_sync();
+    float zme = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const bf16x4 xb = *(const bf16x4*)&xs[(16 * t + s) * 16 + 4 * q];
+        bf16x4 a1[4];
+#pragma unroll
+        for (int hb = 0; hb < 4; ++hb) {
+            const bf16x4 w = *(const bf16x4*)&L.w1[(hb * 16 + s) * 16 + 4 * q];
+            f32x4 c;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[j] = L.b1[hb * 16 + 4 * q + j];
+            c = mfma_bf16(w, xb, c);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a1[hb][j] = f2bf(silu_hw(c[j]));
+        }
+        float zp = 0.0f;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            f32x4 c;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[j] = L.b2[mb * 16 + 4 * q + j];
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const bf16x4 w = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 16 + 4 * q];
+                c = mfma_bf16(w, a1[kb], c);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) zp += L.w3[mb * 16 + 4 * q + j] * silu_hw(c[j]);
+        }
+        zp += __shfl_xor(zp, 16, 64);
+        zp += __shfl_xor(zp, 32, 64);
+        if (q == t) zme = zp;
+    }
+    wave_lds_sync();   // the staging area is reused by the next call
+    return zme + b3;
+}
+
+// Normalise features; true when every one lies inside the training envelope.
+__device__ __forceinline__ bool mlp_normalise(const MlpNorm& N, const float* x, float* xn)
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < NFEAT; ++k) {
+        xn[k] = (x[k] - N.mu[k]) / N.sd[k];
+        ok = ok && xn[k] >= N.lo[k] && xn[k] <= N.hi[k];
+    }
+    return ok;
+}
+
+__device__ __forceinline__ float mlp_coefficient(float z, float c) { return c * softplus_f(z) / LN2F; }
+
+}  // namespace lbm

@@ -86,6 +86,10 @@ def host_collide_lib():
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double]
         L.host_omega_les.restype = ctypes.c_double
+        L.host_strain_features.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        L.host_strain_features.restype = None
+        L.host_cell_features.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.host_cell_features.restype = None
         _hc = L
     return _hc
 

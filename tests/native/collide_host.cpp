@@ -83,3 +83,36 @@ double host_omega_les(int fast, int use_double, uint32_t compat, const double* f
     return omega_les_fast(central2(r, (float)ux, (float)uy, (float)uz), (float)rho, (float)tau, (float)cles);
 }
 }
+
+#include "features.hpp"
+
+extern "C" {
+
+// Closure features of a strain tensor (xx, yy, zz, xy, xz, yz) and of a cell's
+// central second moments, in double or float.
+void host_strain_features(int use_double, const double* s6, double rho, double tau0, double* out12)
+{
+    if (use_double) {
+        double x[NFEAT];
+        strain_features(s6[0], s6[1], s6[2], s6[3], s6[4], s6[5], rho, tau0, x);
+        for (int k = 0; k < NFEAT; ++k) out12[k] = x[k];
+    } else {
+        float x[NFEAT];
+        strain_features((float)s6[0], (float)s6[1], (float)s6[2], (float)s6[3], (float)s6[4], (float)s6[5],
+                        (float)rho, (float)tau0, x);
+        for (int k = 0; k < NFEAT; ++k) out12[k] = x[k];
+    }
+}
+
+void host_cell_features(const double* fin, double tau0, double cles, double* out12, double* pin_rho,
+                        double* omega_s)
+{
+    double f[Q];
+    for (int q = 0; q < Q; ++q) f[q] = fin[q];
+    const Raw2<double> r = raw_moments(f);
+    const double rho = r.m0, ux = r.jx / rho, uy = r.jy / rho, uz = r.jz / rho;
+    const Cen2<double> c = central2(r, ux, uy, uz);
+    cell_features(c, rho, tau0, cles, out12, *pin_rho, *omega_s);
+}
+
+}  // extern "C"

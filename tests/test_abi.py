@@ -27,6 +27,15 @@ def test_core_exports_every_declared_symbol():
     assert sorted(n for n, _, _ in abi.CORE_SYMBOLS) == names
 
 
+def test_core_exports_every_closure_symbol():
+    lib = ctypes.CDLL(abi.CORE_LIB)
+    names = declared("lbm_closure.h")
+    assert "lbm_mlp_train" in names and "lbm_set_closure" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(n for n, _, _ in abi.CLOSURE_SYMBOLS) == names
+
+
 def test_status_strings():
     lib = abi.load_core()
     for s in range(7):
