@@ -157,6 +157,17 @@ lbm_status lbm_get_omega(lbm_ctx* ctx, double* omega);
  * returns LBM_E_NAN if any is non-finite. */
 lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
 
+/* Advance one step and reduce, in FP64 over that step's cells:
+ * out[0..3] = mass and momentum of the state the step starts from (the
+ * moments its collision uses), out[4..6] = momentum the fluid loses across
+ * the half-way bounce-back links of wall-list nodes in this step, i.e. the
+ * force on the walls (drag and lift on an obstacle): for every wall node x
+ * and direction q whose source x - c_q is solid,
+ * F -= c_q (f_bounced(x, q) + f*(x, opp q)).  With the true opposite table
+ * this is the momentum-exchange sum 2 f*(x, k) c_k over links k into the
+ * solid.  (No reference counterpart; the learned closure is excluded.) */
+lbm_status lbm_step_diagnostics(lbm_ctx* ctx, double out[7]);
+
 /* Steps completed so far. */
 lbm_status lbm_step_count(lbm_ctx* ctx, int64_t* steps);
 

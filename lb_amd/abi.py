@@ -63,6 +63,7 @@ CORE_SYMBOLS = [
     ("lbm_probe", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]),
     ("lbm_get_omega", ctypes.c_int, [_vp, _dp]),
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
+    ("lbm_step_diagnostics", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("lbm_last_kernel_ms", ctypes.c_int, [_vp, _dp]),
     ("lbm_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),

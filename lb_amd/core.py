@@ -111,6 +111,12 @@ class Lattice:
         _check(self._lib.lbm_conservation(self._h, abi.ptr(out)), "lbm_conservation")
         return out
 
+    def step_diagnostics(self):
+        """One step; returns (mass, momentum[3], wall force[3]) of that step."""
+        out = np.empty(7)
+        _check(self._lib.lbm_step_diagnostics(self._h, abi.ptr(out)), "lbm_step_diagnostics")
+        return out[0], out[1:4].copy(), out[4:7].copy()
+
     def steps(self) -> int:
         v = ctypes.c_int64()
         _check(self._lib.lbm_step_count(self._h, ctypes.byref(v)), "lbm_step_count")

@@ -56,6 +56,7 @@ struct lbm_ctx {
     bool initialized = false;
     bool first = false;
     bool nan_seen = false;
+    bool diag_next = false;    // the next launched step reduces mass, momentum and wall force
     double last_ms = 0.0;
     size_t dev_bytes = 0;
     // learned closure
@@ -234,7 +235,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         a.first = c->first ? 1 : 0;
-        a.diag = 0;
+        a.diag = (s == 0 && c->diag_next) ? 1 : 0;
         const bool odd = (c->steps & 1) != 0;
         if constexpr (sizeof(R) == 4 && !LIT) {
             if (mlp) {
@@ -755,12 +756,12 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     if (st == LBM_OK) st = dev_alloc(c, &c->macro, 4 * N * rb);
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->stage, 5 * NU * sizeof(double));
     const size_t nparts = std::max((size_t)c->g.ntiles, (size_t)blocks_for(NU));
-    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->partials, nparts * 4 * sizeof(double));
+    if (st == LBM_OK) st = dev_alloc(c, (void**)&c->partials, nparts * 8 * sizeof(double));
     if (st != LBM_OK) {
         lbm_destroy(c);
         return st;
     }
-    c->h_partials.resize(nparts * 4);
+    c->h_partials.resize(nparts * 8);
     auto up = [&](void* dst, const void* src, size_t bytes) {
         return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
     };
@@ -1084,6 +1085,34 @@ lbm_status lbm_device_bytes(lbm_ctx* c, size_t* bytes)
 {
     if (!c || !bytes) return LBM_E_ARG;
     *bytes = c->dev_bytes;
+    return LBM_OK;
+}
+
+lbm_status lbm_step_diagnostics(lbm_ctx* c, double out[7])
+{
+    if (!c || !out) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    if (c->d.closure == LBM_CLOSURE_MLP) return LBM_E_UNSUPPORTED;
+    c->diag_next = true;
+    lbm_status st = lbm_step(c, 1);
+    c->diag_next = false;
+    if (st != LBM_OK) return st;
+    st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    const size_t nb = (size_t)c->g.ntiles;
+    HIPCHK(hipMemcpy(c->h_partials.data(), c->partials, nb * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t b = 0; b < nb; ++b)
+        for (int k = 0; k < 7; ++k) s[k] += c->h_partials[b * 8 + k];
+    bool finite = true;
+    for (int k = 0; k < 7; ++k) {
+        out[k] = s[k];
+        finite = finite && std::isfinite(s[k]);
+    }
+    if (!finite) {
+        c->nan_seen = true;
+        return LBM_E_NAN;
+    }
     return LBM_OK;
 }
 

@@ -92,7 +92,7 @@ struct StepArgs {
     typename ST::R* pub_mac;       // [n_pub][2]: rho, u_x
     typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
     const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
-    double* partials;              // [ntiles][4] when diag != 0
+    double* partials;              // [ntiles][8] when diag != 0: mass, momentum, wall force
     Geo g;
     Consts<typename ST::R> K;
     typename ST::R rest[4];        // w*rho0 per weight class (deviation storage)
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
     uint8_t cls = CL_FLUID;
     if (FLAGS && active) cls = a.cls[c.i];
     if (cls == CL_SOLID) active = false;
-    double dm[4] = {0.0, 0.0, 0.0, 0.0};
+    double dm[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // mass, momentum, wall force
 
     if (active) {
         R f[Q];
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         }
 
         if (!FLAGS || cls == CL_FLUID) {
-    #pragma unroll
+#pragma unroll
             for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
         } else {
             const uint64_t nw = a.node[c.i];
@@ -407,22 +407,22 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             const bool wall = (nw >> 63) != 0;
             if (inlet) {
                 // stream the pre-collision state (StreamingAB.h:62-65)
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < Q; ++q)
                     if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
                 if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                     R st[Q];
-    #pragma unroll
+#pragma unroll
                     for (int q = 0; q < Q; ++q) st[q] = f[q];
                     moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
-    #pragma unroll
+#pragma unroll
                     for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
                 }
             } else {
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < Q; ++q)
                     if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     if ((S >> q) & 1u) {
                         R v;
@@ -433,6 +433,14 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                         else
                             v = P[OPP[q]];
                         store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
+                        if (wall && a.diag) {
+                            // momentum the fluid loses across this link: the
+                            // outgoing f*_{opp q} leaves, the bounced v enters
+                            const double m = (double)v + (double)P[OPP[q]];
+                            dm[4] -= CX[q] * m;
+                            dm[5] -= CY[q] * m;
+                            dm[6] -= CZ[q] * m;
+                        }
                     }
                 }
                 // data the boundary patch kernels of this step read
@@ -440,7 +448,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                     a.pub_mac[(size_t)bi * 2] = rho;
                     a.pub_mac[(size_t)bi * 2 + 1] = ux;
                     R* dst = a.pub_post + (size_t)bi * Q;
-    #pragma unroll
+#pragma unroll
                     for (int q = 0; q < Q; ++q) dst[q] = P[q];
                 }
             }
@@ -449,9 +457,9 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 
     if (a.diag) {
         // wave-level FP64 reduction, then across the 4 waves of the block
-        __shared__ double red[TPB / 64][4];
+        __shared__ double red[TPB / 64][7];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 7; ++k) {
             double v = dm[k];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -460,14 +468,14 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) red[wv][k] = dm[k];
+            for (int k = 0; k < 7; ++k) red[wv][k] = dm[k];
         }
         __syncthreads();
-        if (threadIdx.x < 4) {
+        if (threadIdx.x < 7) {
             double s = 0.0;
 #pragma unroll
             for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
-            a.partials[(size_t)blockIdx.x * 4 + threadIdx.x] = s;
+            a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
         }
     }
 }
