@@ -1,0 +1,182 @@
+"""Measured studies on the GPU (host orchestration over the C ABI).
+
+c3  Taylor-Green vortex 384^3, Re 1600, cumulant: FP64 storage as the
+    reference run against FP32, FP16-deviation and BF16-deviation storage
+    (all FP32 arithmetic except FP64); rel-L2 of u and rho and the
+    kinetic-energy error over 10 000 steps, MLUPS per storage.
+c4  flow past a voxelized obstacle, 512 x 256 x 256: synthetic OFF scene,
+    the host voxelizer and boundary mesher, equilibrium velocity inlet at
+    x = 0, interpolated density outlet at x = 511, index wrap along y and z,
+    cumulant + Smagorinsky; drag from the wall-link momentum exchange, FP32
+    against FP64 storage, MLUPS.
+
+usage: python -m lb_amd.studies c3|c4 [--steps N] [--out file.json]
+"""
+from __future__ import annotations
+
+import json
+import os
+import tempfile
+import time
+
+import numpy as np
+
+from . import abi, host
+from .core import Lattice, make_desc
+from .fields import taylor_green
+from .geometry import obstacle_scene, write_off
+
+
+def _log(msg):
+    print(f"[study] {msg}", flush=True)
+
+
+def _rel(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b * b)))
+
+
+def c3(n=384, steps=10000, every=1000, u0=0.05, storages=("f64", "f32", "f16dev", "bf16dev")):
+    L = n / (2.0 * np.pi)
+    nu = u0 * L / 1600.0
+    tau = 3.0 * nu + 0.5
+    rho0, ux, uy, uz = taylor_green(n, u0)
+    flags = np.ones((n, n, n), np.int32)
+    codes = {"f64": abi.STORAGE_F64, "f32": abi.STORAGE_F32, "f16dev": abi.STORAGE_F16DEV,
+             "bf16dev": abi.STORAGE_BF16DEV}
+    lats = {}
+    for s in storages:
+        lats[s] = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, storage=codes[s], arith=abi.ARITH_FAST,
+                                    compat=0, periodic=(1, 1, 1), tau=tau))
+        lats[s].init_equilibrium(rho0, ux, uy, uz)
+    del rho0, ux, uy, uz
+    res = {"workload": f"Taylor-Green {n}^3, u0={u0}, Re=1600, nu={nu:.6g}, tau={tau:.6g}, cumulant-2015 fast",
+           "steps": steps, "records": []}
+    ms = {s: [] for s in storages}
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        for s in storages:
+            lats[s].step(k)
+            lats[s].sync()
+            ms[s].append(lats[s].last_kernel_ms())
+        done += k
+        ref = lats["f64"].macroscopic()
+        Kref = 0.5 * float(np.mean(ref[1] ** 2 + ref[2] ** 2 + ref[3] ** 2))
+        rec = {"step": done, "K_f64": Kref}
+        for s in storages:
+            if s == "f64":
+                continue
+            m = lats[s].macroscopic()
+            K = 0.5 * float(np.mean(m[1] ** 2 + m[2] ** 2 + m[3] ** 2))
+            rec[s] = {"rel_l2_u": _rel(np.stack(m[1:]), np.stack(ref[1:])), "rel_l2_rho": _rel(m[0], ref[0]),
+                      "rel_l2_rho_minus_1": _rel(m[0] - 1.0, ref[0] - 1.0), "K_rel_err": abs(K - Kref) / Kref}
+            del m
+        res["records"].append(rec)
+        _log(f"step {done}: " + ", ".join(f"{s} u {rec[s]['rel_l2_u']:.2e}" for s in storages if s != "f64"))
+        del ref
+    res["mlups"] = {s: n ** 3 / (float(np.median(ms[s])) * 1e-3) / 1e6 for s in storages}
+    res["ms_per_step"] = {s: float(np.median(ms[s])) for s in storages}
+    for s in storages:
+        lats[s].close()
+    return res
+
+
+def c4_geometry(workdir, res=1.0, add=0.0):
+    """Voxelized flag grid and boundary list of the obstacle configuration."""
+    v, f, objs = obstacle_scene()
+    path = os.path.join(workdir, "obstacle.off")
+    write_off(path, v, f)
+    t0 = time.time()
+    grid, mn, mx = host.voxelize_off(path, res, add, (-3.1416, -2.7183, -1.4142))
+    tv = time.time() - t0
+    nz, ny, nx = grid.shape
+    m = host.Mesher(grid, res, add, mn)
+    m.wall()
+    big = 1e4
+    # cell centres sit at i + 0.5: the planes x = 0 and x = nx - 1
+    m.inlet((0.2, -big, -big), (0.8, big, big), (0.8, big, -big), -1, (-1.0, 0.0, 0.0), (0.04, 0.0, 0.0),
+            abi.BC_INLET_EQUILIBRIUM)
+    m.outlet((nx - 0.8, -big, -big), (nx - 0.2, big, big), (nx - 0.2, big, -big), -2, (1.0, 0.0, 0.0), 1.0)
+    flags, bc = m.compile()
+    # analytic check of the voxelization away from the surfaces
+    z, y, x = [a + 0.5 for a in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij", sparse=True)]
+    inside_any = np.zeros(grid.shape, bool)
+    near = np.zeros(grid.shape, bool)
+    for _, c, r in objs:
+        s = np.sqrt(((x - c[0]) / r[0]) ** 2 + ((y - c[1]) / r[1]) ** 2 + ((z - c[2]) / r[2]) ** 2)
+        inside_any |= s < 1
+        near |= np.abs(s - 1) < 2.0 / r.min()
+    far = ~near
+    mism = int(np.count_nonzero(grid[far].astype(bool) == inside_any[far]))
+    info = {"dims": [nx, ny, nz], "triangles": int(len(f)), "voxelize_s": tv, "fluid_cells": int(grid.sum()),
+            "solid_cells": int(grid.size - grid.sum()), "analytic_mismatch_far": mism,
+            "near_surface_cells": int(near.sum()), "wall_nodes": int(sum(1 for b in bc if b.kind == abi.BC_WALL)),
+            "inlet_nodes": int(sum(1 for b in bc if b.kind == abi.BC_INLET_EQUILIBRIUM)),
+            "outlet_nodes": int(sum(1 for b in bc if b.kind == abi.BC_OUTLET_INTERPOLATED)),
+            "obstacles": [[k, [float(t) for t in c], [float(t) for t in r]] for k, c, r in objs]}
+    return flags, bc, info
+
+
+def c4(steps=5000, every=250, fp64_steps=None):
+    D = 48.0
+    u_in = 0.04
+    nu = u_in * D / 1000.0
+    tau = 3.0 * nu + 0.5
+    with tempfile.TemporaryDirectory() as td:
+        flags, bc, info = c4_geometry(td)
+    _log(f"geometry {info['dims']} voxelized in {info['voxelize_s']:.1f} s, far-field mismatches "
+         f"{info['analytic_mismatch_far']}, walls {info['wall_nodes']}")
+    res = {"workload": f"obstacle 512x256x256, inlet u={u_in}, outlet rho=1, y/z wrap, Re_D=1000 nu={nu:.4g}, "
+                       "cumulant-2015 + Smagorinsky C=0.17", "geometry": info}
+    fp64_steps = fp64_steps or steps
+    ux0 = np.where(flags == 0, 0.0, u_in)
+    for name, st, nst in (("f32", abi.STORAGE_F32, steps), ("f64", abi.STORAGE_F64, fp64_steps)):
+        lat = Lattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+                                storage=st, arith=abi.ARITH_FAST, compat=abi.COMPAT_REFERENCE, periodic=(0, 1, 1),
+                                tau=tau, nu=nu, c_les=0.17))
+        lat.init_equilibrium(None, ux0, None, None)
+        t, F, ms = [], [], []
+        done = 0
+        t0 = time.time()
+        while done < nst:
+            k = min(every - 1, nst - done - 1)
+            if k > 0:
+                lat.step(k)
+                lat.sync()
+                ms.append(lat.last_kernel_ms())
+            _, _, force = lat.step_diagnostics()
+            done += k + 1
+            t.append(done)
+            F.append(force.tolist())
+        _log(f"{name}: {nst} steps in {time.time() - t0:.1f} s, drag {F[-1][0]:.4e}")
+        res[name] = {"steps": t, "force": F, "ms_per_step": float(np.median(ms)),
+                     "mlups": float(info["fluid_cells"] / (np.median(ms) * 1e-3) / 1e6),
+                     "mlups_all_cells": float(flags.size / (np.median(ms) * 1e-3) / 1e6)}
+        lat.close()
+    n = min(len(res["f32"]["force"]), len(res["f64"]["force"]))
+    f32 = np.array(res["f32"]["force"][:n])
+    f64 = np.array(res["f64"]["force"][:n])
+    half = n // 2
+    res["drag_rel_diff_f32_vs_f64"] = float(np.mean(np.abs(f32[half:, 0] - f64[half:, 0])) /
+                                            np.mean(np.abs(f64[half:, 0])))
+    res["drag_coefficient_f64"] = float(np.mean(f64[half:, 0]) / (0.5 * u_in ** 2 * np.pi * (D / 2) ** 2))
+    return res
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["c3", "c4"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--n", type=int, default=384)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    if a.study == "c3":
+        r = c3(n=a.n, steps=a.steps or 10000)
+    else:
+        r = c4(steps=a.steps or 5000)
+    if a.out:
+        with open(a.out, "w") as fh:
+            json.dump(r, fh, indent=1)
+    print(json.dumps({k: v for k, v in r.items() if k not in ("records",)})[:4000])

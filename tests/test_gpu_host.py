@@ -1,0 +1,73 @@
+"""The C++ host driver (include/lbm_host.h) end to end on the GPU: a lid-driven
+cavity set up through the case API (grid, wall meshing, moving-wall lid,
+compile, initialise, run) against the oracle run from the numpy mesher's
+flags and lists (the two meshers agree, tests/test_host.py)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import OracleLattice, cavity_setup
+from lb_amd import abi, host, make_desc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_case_cavity_run_matches_oracle(gpu, tmp_path):
+    n, steps, tau = 10, 60, 0.56
+    L = host.load_host()
+    h = ctypes.c_void_p()
+    assert L.lbmh_case_create(ctypes.byref(h)) == 0
+    try:
+        grid = np.zeros((n + 2, n + 2, n + 2), np.int32)
+        grid[1:-1, 1:-1, 1:-1] = 1
+        dims = (ctypes.c_int32 * 3)(n + 2, n + 2, n + 2)
+        assert L.lbmh_case_set_grid(h, dims, grid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 1.0, 1.0,
+                                    (ctypes.c_float * 3)(0, 0, 0), (ctypes.c_float * 3)(n, n, n)) == 0
+        for k, v in (("tau", tau), ("ny", (tau - 0.5) / 3), ("rho_fyz", 1.0), ("Cm", 1.0), ("Cl", 1.0),
+                     ("Cu_inverse", 1.0), ("iterations", steps), ("err_every_it", 20), ("plot_every_it", -1),
+                     ("probe_every_it", -1), ("collision", abi.COLLISION_CUM2015), ("closure", abi.CLOSURE_NONE),
+                     ("storage", abi.STORAGE_F64), ("arith", abi.ARITH_LITERAL), ("compat", abi.COMPAT_REFERENCE)):
+            assert L.lbmh_case_set(h, k.encode(), float(v)) == 0, k
+        assert L.lbmh_case_set_results_dir(h, str(tmp_path).encode()) == 0
+        assert L.lbmh_case_mesh_wall(h) == 0
+        lid = (ctypes.c_float * 3)(float(np.float32(0.05)), 0.0, 0.0)
+        assert L.lbmh_case_inlet(h, (ctypes.c_double * 3)(-1.0, n - 0.75, -1.0),
+                                 (ctypes.c_double * 3)(n + 1.0, n - 0.25, n + 1.0),
+                                 (ctypes.c_double * 3)(-1.0, n - 0.75, n + 1.0), -1, (ctypes.c_float * 3)(0, 1, 0),
+                                 lid, abi.BC_INLET_MOVING_WALL) == 0
+        assert L.lbmh_case_compile(h) == 0
+        assert L.lbmh_case_initialize(h) == 0
+        assert L.lbmh_case_run(h) == 0
+        flags = np.zeros((n + 2,) * 3, np.int32)
+        assert L.lbmh_case_flags(h, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) == 0
+        ctx = L.lbmh_case_lattice(h)
+        lib = abi.load_core()
+        out = [np.empty((n + 2,) * 3) for _ in range(4)]
+        assert lib.lbm_get_macroscopic(ctx, *[abi.ptr(a) for a in out]) == 0
+        errs = (ctypes.c_double * 8)()
+        ne = ctypes.c_int32(8)
+        assert L.lbmh_case_errors(h, errs, ctypes.byref(ne)) == 0
+    finally:
+        L.lbmh_case_destroy(h)
+    oflags, bc, ux = cavity_setup(n)
+    assert np.array_equal(flags, oflags)
+    # LBMConstants fields are float (RealType), so the case runs with float(tau)
+    o = OracleLattice(make_desc(oflags, bc, collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
+                                arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=float(np.float32(tau))))
+    o.init_equilibrium(None, ux, None, None)
+    # the reference loop evaluates the error at the end of iteration k with
+    # k % err_every_it == 0, k != 0, i.e. after k + 1 steps (SolverTurbulentLES.h:184-251)
+    ref_err = []
+    o.step(21)
+    ref_err.append(o.error_quadratic())
+    o.step(20)
+    ref_err.append(o.error_quadratic())
+    o.step(steps - 41)
+    mo = o.macroscopic()
+    for a, b in zip(out, mo):
+        assert np.array_equal(a, b)
+    got = [errs[i] for i in range(ne.value)]
+    print(f"[host case] errors {got} oracle {ref_err}", flush=True)
+    assert len(got) >= 2
+    assert np.allclose(got, ref_err[:len(got)], rtol=1e-6)
