@@ -18,6 +18,7 @@ workload, on this host's cores.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -52,6 +53,22 @@ def profiled_traffic(storage_name, n):
     return d["traffic_bytes_per_launch"]["mean"], os.path.relpath(path, ROOT)
 
 
+def closure_result():
+    """Dissipation and spectrum errors of the learned closure against the
+    filtered fine run, from the committed experiment summary."""
+    path = os.path.join(ROOT, "profiles", "closure_c5_512to256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    e = d["eval"]
+    return {"source": os.path.relpath(path, ROOT),
+            "dissipation_rel_err": {"smagorinsky": round(e["smagorinsky"]["dissipation_rel_err"], 4),
+                                    "mlp": round(e["mlp"]["dissipation_rel_err"], 4)},
+            "spectrum_log_rms": {"smagorinsky": round(e["smagorinsky"]["spectrum_log_rms"], 3),
+                                 "mlp": round(e["mlp"]["spectrum_log_rms"], 3)}}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -65,8 +82,28 @@ def make_lattice(n, storage, fields, arith=abi.ARITH_FAST):
     return L
 
 
-def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST):
-    L = make_lattice(n, storage, fields, arith)
+def closure_network(seed=7):
+    """A seeded closure network (untrained weights, non-zero output layer) and
+    an open envelope: the step cost does not depend on the weights."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(abi.MLP_PARAMS, np.float32)
+    p[:768] = rng.standard_normal(768) / np.sqrt(12.0)
+    p[832:4928] = rng.standard_normal(4096) / 8.0
+    p[4992:5056] = 0.05 * rng.standard_normal(64)
+    norm = np.stack([np.zeros(12), np.ones(12), np.full(12, -1e30), np.full(12, 1e30)]).astype(np.float32)
+    return p, norm
+
+
+def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False):
+    if mlp:
+        flags = np.ones((n, n, n), np.int32)
+        L = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_MLP, storage=storage,
+                              arith=abi.ARITH_FAST, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=TAU0,
+                              c_les=C_LES))
+        L.set_closure(*closure_network())
+        L.init_equilibrium(None, *fields)
+    else:
+        L = make_lattice(n, storage, fields, arith)
     L.step(warmup)
     L.sync()
     t0 = time.perf_counter()
@@ -126,9 +163,8 @@ def main():
     if n_dev <= 0:
         raise SystemExit(f"no HIP device visible: {dev}")
     log(f"device: {dev}")
-    probe_gbs, _ = 0.0, None
+    probe_gbs = 0.0
     g = abi.load_core()
-    import ctypes
     gb, ms = ctypes.c_double(), ctypes.c_double()
     if g.lbm_bandwidth_probe(1 << 30, 20, ctypes.byref(gb), ctypes.byref(ms)) == abi.LBM_OK:
         probe_gbs = gb.value
@@ -170,6 +206,13 @@ def main():
                               "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                               "steps": vsteps, "bytes_per_cell": 54 * sb}
             log(f"{name}: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
+        # learned closure fused into the same step (FP32 storage)
+        r = run_gpu(n, abi.STORAGE_F32, fields, vsteps, 20, mlp=True)
+        variants["f32_mlp_closure"] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                                       "steps": vsteps, "cost_vs_smagorinsky": round(res["kernel_ms"] and
+                                                                                      r["kernel_ms"] /
+                                                                                      res["kernel_ms"] - 1, 4)}
+        log(f"f32 + learned closure: {r['mlups']:.0f} MLUPS ({variants['f32_mlp_closure']['cost_vs_smagorinsky']:+.1%})")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -196,6 +239,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
+        "closure_error": closure_result(),
     }
     print(json.dumps(out), flush=True)
 
