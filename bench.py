@@ -94,13 +94,14 @@ def closure_network(seed=7):
     return p, norm
 
 
-def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False):
+def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False, refresh=1):
     if mlp:
         flags = np.ones((n, n, n), np.int32)
         L = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_MLP, storage=storage,
                               arith=abi.ARITH_FAST, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=TAU0,
                               c_les=C_LES))
         L.set_closure(*closure_network())
+        L.set_closure_refresh(refresh)
         L.init_equilibrium(None, *fields)
     else:
         L = make_lattice(n, storage, fields, arith)
@@ -206,13 +207,16 @@ def main():
                               "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                               "steps": vsteps, "bytes_per_cell": 54 * sb}
             log(f"{name}: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
-        # learned closure fused into the same step (FP32 storage)
-        r = run_gpu(n, abi.STORAGE_F32, fields, vsteps, 20, mlp=True)
-        variants["f32_mlp_closure"] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
-                                       "steps": vsteps, "cost_vs_smagorinsky": round(res["kernel_ms"] and
-                                                                                      r["kernel_ms"] /
-                                                                                      res["kernel_ms"] - 1, 4)}
-        log(f"f32 + learned closure: {r['mlups']:.0f} MLUPS ({variants['f32_mlp_closure']['cost_vs_smagorinsky']:+.1%})")
+        # learned closure fused into the same step (FP32 storage), network
+        # evaluated every step and every 10 steps (coefficient held between)
+        for every in (1, 10):
+            r = run_gpu(n, abi.STORAGE_F32, fields, vsteps, 20, mlp=True, refresh=every)
+            key = "f32_mlp_closure" if every == 1 else f"f32_mlp_closure_refresh{every}"
+            variants[key] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5), "steps": vsteps,
+                             "refresh": every,
+                             "cost_vs_smagorinsky": round(r["kernel_ms"] / res["kernel_ms"] - 1, 4)}
+            log(f"f32 + learned closure (every {every}): {r['mlups']:.0f} MLUPS "
+                f"({variants[key]['cost_vs_smagorinsky']:+.1%})")
 
     cpu = None
     if not args.no_cpu_baseline:

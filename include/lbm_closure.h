@@ -67,9 +67,17 @@ void lbm_mlp_destroy(lbm_mlp* m);
  * LBM_E_STATE. */
 lbm_status lbm_set_closure(lbm_ctx* ctx, const float* params, const float* norm);
 
+/* Evaluate the network every `every` steps (default 1) and hold each cell's
+ * coefficient C(x) in between: the steps in between still apply the rate
+ * formula to the cell's current |Pi_neq| / rho, only C is held.  The
+ * network's coefficient moves on the turbulence time scale, many steps long.
+ * Restarts the cycle: the next step evaluates the network. */
+lbm_status lbm_set_closure_refresh(lbm_ctx* ctx, int32_t every);
+
 /* Per cell of the current state (caller layout): normalised features
  * [12][N] (may be NULL), envelope flag [N] (may be NULL) and the coefficient
- * the next step uses [N] (c_les outside the envelope; may be NULL).
+ * the next step uses [N] (c_les outside the envelope, the held value when the
+ * next step does not evaluate the network; may be NULL).
  * Requires LBM_CLOSURE_MLP and an installed network.  Synchronises. */
 lbm_status lbm_get_closure_features(lbm_ctx* ctx, float* features, int32_t* in_envelope, double* coefficient);
 

@@ -92,6 +92,7 @@ CLOSURE_SYMBOLS = [
     ("lbm_mlp_set_params", ctypes.c_int, [_vp, _fp]),
     ("lbm_mlp_destroy", None, [_vp]),
     ("lbm_set_closure", ctypes.c_int, [_vp, _fp, _fp]),
+    ("lbm_set_closure_refresh", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("lbm_get_closure_features", ctypes.c_int, [_vp, _fp, _ip, _dp]),
     ("lbm_filter", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
 ]

@@ -221,9 +221,10 @@ def kinetic_energy(u, v, w):
     return 0.5 * float(np.mean(u * u + v * v + w * w))
 
 
-def les_run(filtered0, nu, closure, steps, every, c_les=0.17, params=None, norm=None):
+def les_run(filtered0, nu, closure, steps, every, c_les=0.17, params=None, norm=None, refresh=1):
     """Coarse LES from a filtered field: returns (steps, K(t)) and the final
-    velocity, with closure 'smagorinsky' or 'mlp'."""
+    velocity, with closure 'smagorinsky' or 'mlp' (network evaluated every
+    `refresh` steps)."""
     n = filtered0.shape[1]
     flags = np.ones((n, n, n), np.int32)
     cl = abi.CLOSURE_MLP if closure == "mlp" else abi.CLOSURE_SMAGORINSKY
@@ -231,6 +232,7 @@ def les_run(filtered0, nu, closure, steps, every, c_les=0.17, params=None, norm=
                             arith=abi.ARITH_FAST, compat=0, periodic=(1, 1, 1), tau=3.0 * nu + 0.5, c_les=c_les))
     if closure == "mlp":
         lat.set_closure(params, norm)
+        lat.set_closure_refresh(refresh)
     lat.init_equilibrium(filtered0[0], filtered0[1], filtered0[2], filtered0[3])
     ts, K = [], []
     done = 0
@@ -244,8 +246,10 @@ def les_run(filtered0, nu, closure, steps, every, c_les=0.17, params=None, norm=
         lat.step(min(every, steps - done))
         done = min(done + every, steps)
     if closure == "mlp":
-        _, env, _ = lat.closure_features()
+        feats, env, _ = lat.closure_features()
         frac = float(env.mean())
+        out_of = [float(np.mean((feats[k] < norm[2][k]) | (feats[k] > norm[3][k]))) for k in range(NF)]
+        _log("outside the envelope per feature: " + " ".join(f"{v:.3f}" for v in out_of))
     t0 = time.time()
     lat.step(200)
     lat.sync()
@@ -262,7 +266,8 @@ def dissipation(t, K):
 
 def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_seed=20240601, nu_fine=4e-3,
                    train_steps=3000, train_snaps=(1000, 2000, 3000), eval_steps=4000, eval_every=100, batch=32768,
-                   adam_steps=3000, lr=1e-3, c_les=0.17, seed=7, out_json=None):
+                   adam_steps=3000, lr=1e-3, c_les=0.17, seed=7, refresh=10, extra_envelopes=((0.0, 1.0),),
+                   out_json=None):
     """The learned-closure experiment: data from a fine run filtered by 4,
     training, then LES at fine_n/2 against the fine run filtered by 2."""
     res = {"fine_n": fine_n, "nu_fine": nu_fine, "c_les": c_les, "batch": batch}
@@ -312,8 +317,9 @@ def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_
     fine_K = np.array(fine_K)
     les_steps = eval_steps // f_eval
     out = {}
-    for name in ("smagorinsky", "mlp"):
-        t, K, fin, ms, frac = les_run(filt0, nu_e, name, les_steps, eval_every, c_les, params, norm)
+    runs = (("smagorinsky", "smagorinsky", 1), ("mlp", "mlp", 1), (f"mlp_refresh{refresh}", "mlp", refresh))
+    for name, kind, every in runs:
+        t, K, fin, ms, frac = les_run(filt0, nu_e, kind, les_steps, eval_every, c_les, params, norm, every)
         eps = dissipation(t, K)
         eps_ref = dissipation(fine_t, fine_K)
         # compare after the initial transient (first tenth of the run)
@@ -327,9 +333,25 @@ def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_
                      "K_final": float(K[-1]), "in_envelope": frac}
         _log(f"{name}: dissipation rel err {rel:.3f}, spectrum log-rms {spec:.3f}, {ms:.3f} ms/step"
              + (f", envelope {frac:.3f}" if frac is not None else ""))
+    # the same network with wider training envelopes (fewer Smagorinsky fallbacks)
+    for q in extra_envelopes:
+        nq = fit_normalisation(X, q=q)
+        t, K, fin, ms, frac = les_run(filt0, nu_e, "mlp", les_steps, eval_every, c_les, params, nq, refresh)
+        eps = dissipation(t, K)
+        eps_ref = dissipation(fine_t, fine_K)
+        sel = t >= 0.1 * les_steps
+        rel = float(np.mean(np.abs(eps[sel] - eps_ref[sel]) / np.abs(eps_ref[sel])))
+        kl, El = energy_spectrum(*fin)
+        spec = float(np.sqrt(np.mean((np.log(El[1:kmax]) - np.log(Ef[1:kmax])) ** 2)))
+        key = f"mlp_refresh{refresh}_envelope_q{q[0]:g}"
+        out[key] = {"dissipation_rel_err": rel, "spectrum_log_rms": spec, "ms_per_step": ms, "K_final": float(K[-1]),
+                    "in_envelope": frac, "envelope_quantiles": list(q)}
+        _log(f"{key}: dissipation rel err {rel:.3f}, spectrum log-rms {spec:.3f}, envelope {frac:.3f}")
     out["fine_K_final"] = float(fine_K[-1])
     res["eval"] = out
     res["mlups_cost"] = out["mlp"]["ms_per_step"] / out["smagorinsky"]["ms_per_step"] - 1.0
+    res["refresh"] = refresh
+    res["mlups_cost_refresh"] = out[f"mlp_refresh{refresh}"]["ms_per_step"] / out["smagorinsky"]["ms_per_step"] - 1.0
     if out_json:
         with open(out_json, "w") as fh:
             json.dump(res, fh, indent=1)
@@ -346,9 +368,10 @@ if __name__ == "__main__":
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--eval-steps", type=int, default=4000)
     ap.add_argument("--adam-steps", type=int, default=3000)
+    ap.add_argument("--refresh", type=int, default=10)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     snaps = tuple(int(a.train_steps * k / 3) for k in (1, 2, 3))
     r = run_experiment(fine_n=a.fine, train_steps=a.train_steps, train_snaps=snaps, eval_steps=a.eval_steps,
-                       adam_steps=a.adam_steps, out_json=a.out)
+                       adam_steps=a.adam_steps, refresh=a.refresh, out_json=a.out)
     print(json.dumps(r))

@@ -143,6 +143,10 @@ class Lattice:
         _check(self._lib.lbm_set_closure(self._h, abi.ptr(p, ctypes.c_float), abi.ptr(nm, ctypes.c_float)),
                "lbm_set_closure")
 
+    def set_closure_refresh(self, every: int):
+        """Evaluate the closure network every `every` steps, holding C(x) between."""
+        _check(self._lib.lbm_set_closure_refresh(self._h, int(every)), "lbm_set_closure_refresh")
+
     def closure_features(self):
         """(normalised features [12, nz, ny, nx], envelope flag, coefficient)."""
         f = np.empty((abi.MLP_FEATURES, *self.shape), np.float32)

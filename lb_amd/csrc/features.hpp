@@ -54,12 +54,25 @@ LBM_HD void strain_features(R sxx, R syy, R szz, R sxy, R sxz, R syz, R rho, R t
     x[11] = dn * inv;
 }
 
+// |Pi_neq| / rho from the central second moments (the same terms as
+// cell_features and omega_les_fast).
+template <class R>
+LBM_HD R pin_over_rho(const Cen2<R>& c, R rho)
+{
+    LBM_FP_STRICT
+    const R third = rho * R(1.0 / 3.0);
+    const R pxx = c.kxx - third, pyy = c.kyy - third, pzz = c.kzz - third;
+    const R n2 = pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (c.kxy * c.kxy + c.kxz * c.kxz + c.kyz * c.kyz);
+    return sqrt_r(n2) / rho;
+}
+
 // Features of a lattice cell from its central second moments.  Also returns
 // |Pi_neq| / rho (the argument the rate formula needs) and the Smagorinsky
 // rate omega_s (the fallback).
 template <class R>
 LBM_HD void cell_features(const Cen2<R>& c, R rho, R tau0, R cles, R* x, R& pin_rho, R& omega_s)
 {
+    LBM_FP_STRICT
     const R third = rho * R(1.0 / 3.0);
     const R pxx = c.kxx - third, pyy = c.kyy - third, pzz = c.kzz - third;
     const R n2 = pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (c.kxy * c.kxy + c.kxz * c.kxz + c.kyz * c.kyz);

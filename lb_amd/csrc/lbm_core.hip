@@ -65,6 +65,9 @@ struct lbm_ctx {
     uint16_t* cl_wb = nullptr;    // BF16 W1 [64][16] + W2 [64][64]
     float cl_b3 = 0.0f;
     MlpNorm cl_norm{};
+    float* ccache = nullptr;      // coefficient per cell held between network evaluations
+    int cl_every = 1;             // evaluate the network every cl_every steps
+    int64_t cl_count = 0;         // steps since the last (re)start of the refresh cycle
 };
 
 #define HIPCHK(x)                                   \
@@ -217,6 +220,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.pub_post = (R*)c->pub_post;
     a.macro = (const R*)c->macro;
     a.partials = c->partials;
+    a.ccache = c->ccache;
     a.g = c->g;
     a.omega0 = R(1) / (R)c->d.tau;   // 1.f / tau (OmegaNo.h:44, NonDim...h:114)
     a.tau = (R)c->d.tau;
@@ -238,7 +242,8 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         a.diag = (s == 0 && c->diag_next) ? 1 : 0;
         const bool odd = (c->steps & 1) != 0;
         if constexpr (sizeof(R) == 4 && !LIT) {
-            if (mlp) {
+            if (mlp && c->cl_count % c->cl_every == 0) {
+                // network evaluated in this step; coefficients kept for the next ones
                 ma.s = a;
                 if (c->flags_needed) {
                     if (odd) k_step_mlp<ST, COLL, true, true><<<grid, block, 0, c->stream>>>(ma);
@@ -247,7 +252,16 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
                     if (odd) k_step_mlp<ST, COLL, true, false><<<grid, block, 0, c->stream>>>(ma);
                     else k_step_mlp<ST, COLL, false, false><<<grid, block, 0, c->stream>>>(ma);
                 }
+            } else if (mlp) {
+                if (c->flags_needed) {
+                    if (odd) k_step<ST, COLL, false, true, true, true><<<grid, block, 0, c->stream>>>(a);
+                    else k_step<ST, COLL, false, false, true, true><<<grid, block, 0, c->stream>>>(a);
+                } else {
+                    if (odd) k_step<ST, COLL, false, true, false, true><<<grid, block, 0, c->stream>>>(a);
+                    else k_step<ST, COLL, false, false, false, true><<<grid, block, 0, c->stream>>>(a);
+                }
             }
+            if (mlp) c->cl_count++;
         }
         if (!mlp) {
             if (c->flags_needed) {
@@ -641,7 +655,7 @@ void lbm_destroy(lbm_ctx* c)
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
-                    c->cl_wb};
+                    c->cl_wb,    c->ccache};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -807,6 +821,7 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* ux,
     c->initialized = true;
     c->first = true;
     c->nan_seen = false;
+    c->cl_count = 0;   // the next step evaluates the closure network
     return LBM_OK;
 }
 
@@ -829,6 +844,7 @@ lbm_status lbm_set_populations(lbm_ctx* c, const double* f)
     HIPCHK(hipStreamSynchronize(c->stream));
     c->initialized = true;
     c->first = false;
+    c->cl_count = 0;   // the next step evaluates the closure network
     return LBM_OK;
 }
 
@@ -892,6 +908,7 @@ static lbm_status run_closure_eval(lbm_ctx* c, float* feats, int32_t* env, doubl
             e.env = env;
             e.coef = coef;
             e.omega = omega;
+            e.ccache = (c->cl_count % c->cl_every == 0) ? nullptr : c->ccache;
             k_closure_eval<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(e);
             HIPCHK(hipGetLastError());
             return LBM_OK;
@@ -1143,9 +1160,23 @@ lbm_status lbm_set_closure(lbm_ctx* c, const float* params, const float* norm)
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->cl_params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->cl_wb, wb.data(), sizeof(uint16_t) * wb.size(), hipMemcpyHostToDevice));
+    if (!c->ccache) {
+        lbm_status st = dev_alloc(c, (void**)&c->ccache, sizeof(float) * c->N);
+        if (st != LBM_OK) return st;
+    }
     c->cl_b3 = params[OFF_B3];
     c->cl_norm = N;
+    c->cl_count = 0;
     c->mlp_ready = true;
+    return LBM_OK;
+}
+
+lbm_status lbm_set_closure_refresh(lbm_ctx* c, int32_t every)
+{
+    if (!c || every < 1) return LBM_E_ARG;
+    if (c->d.closure != LBM_CLOSURE_MLP) return LBM_E_STATE;
+    c->cl_every = every;
+    c->cl_count = 0;
     return LBM_OK;
 }
 

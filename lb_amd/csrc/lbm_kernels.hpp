@@ -93,6 +93,7 @@ struct StepArgs {
     typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
     const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
     double* partials;              // [ntiles][8] when diag != 0: mass, momentum, wall force
+    float* ccache;                 // learned closure: coefficient per cell, held between refreshes
     Geo g;
     Consts<typename ST::R> K;
     typename ST::R rest[4];        // w*rho0 per weight class (deviation storage)
@@ -316,7 +317,7 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
     }
 }
 
-template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS>
+template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false>
 __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 {
     using R = typename ST::R;
@@ -381,7 +382,10 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 uz = r.jz * ir;
             }
             const Cen2<R> ce = central2(r, ux, uy, uz);
-            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+            if (CC)   // learned closure between refreshes: held coefficient, current stress
+                omega = omega_from_coefficient((R)a.ccache[c.i], pin_over_rho(ce, rho), a.tau);
+            else
+                omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
             if (COLL == LBM_COLLISION_CUM2015) {
                 cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
             } else {
@@ -564,6 +568,7 @@ __global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
     const float z = mlp_wave_forward(L, xn, ma.mlp.b3);
     if (active) {
         const R omega = cp.env ? omega_from_coefficient(mlp_coefficient(z, ma.mlp.c), cp.pin, a.tau) : cp.om_s;
+        if (a.ccache) a.ccache[c.i] = cp.env ? mlp_coefficient(z, ma.mlp.c) : (float)a.cles;
         R P[Q];
         if (COLL == LBM_COLLISION_CUM2015) {
             cum2015_fast(a.K, cp.ce, cp.m0, cp.rho, cp.ux, cp.uy, cp.uz, omega, P);
@@ -1027,6 +1032,7 @@ struct ClosureEvalArgs {
     int32_t* env;     // [NU], or null
     double* coef;     // [NU] coefficient the next step uses, or null
     double* omega;    // [NU] rate the next step uses, or null
+    const float* ccache;  // held coefficients when the next step does not refresh, or null
 };
 
 // Closure inputs and outputs of the current state, per caller cell; the
@@ -1064,10 +1070,13 @@ __global__ __launch_bounds__(TPB) void k_closure_eval(ClosureEvalArgs<ST> e)
     if (e.feats)
         for (int k = 0; k < NFEAT; ++k) e.feats[k * NU + iu] = active ? cp.xn[k] : 0.0f;
     if (e.env) e.env[iu] = (active && cp.env) ? 1 : 0;
-    const float C = cp.env ? mlp_coefficient(z, e.mlp.c) : (float)a.cles;
+    const float C = e.ccache ? e.ccache[c.i] : (cp.env ? mlp_coefficient(z, e.mlp.c) : (float)a.cles);
     if (e.coef) e.coef[iu] = active ? (double)C : 0.0;
-    if (e.omega)
-        e.omega[iu] = active ? (double)(cp.env ? omega_from_coefficient(C, cp.pin, a.tau) : cp.om_s) : 0.0;
+    if (e.omega) {
+        const R om = e.ccache ? omega_from_coefficient((R)C, pin_over_rho(cp.ce, cp.rho), a.tau)
+                              : (cp.env ? omega_from_coefficient(C, cp.pin, a.tau) : cp.om_s);
+        e.omega[iu] = active ? (double)om : 0.0;
+    }
 }
 
 // Box filter over fct^3 blocks of caller cells: averages of rho, u and u u

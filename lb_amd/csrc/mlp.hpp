@@ -1,11 +1,14 @@
 // mlp.hpp -- the closure network on CDNA4 matrix cores (device code).
 //
 // Network: 12 -> 64 -> 64 -> 1, SiLU hidden activations, BF16 weights and
-// activations into v_mfma_f32_16x16x16_bf16, FP32 accumulation; output head
+// activations on the matrix cores (v_mfma_f32_16x16x16_bf16 for layer 1,
+// v_mfma_f32_16x16x32_bf16 for layer 2), FP32 accumulation; output head
 // C = c_les * softplus(z) / ln 2 (z = 0 gives the Smagorinsky constant).
 //
 // Matrix-core layout (16x16x16, one wave): D[16x16] = A[16x16] B[16x16] + C;
-// lane l = 16 q + s holds A[s][4q+j], B[4q+j][s] and D[4q+j][s], j = 0..3.
+// lane l = 16 q + s holds A[s][4q+j], B[4q+j][s] and D[4q+j][s], j = 0..3
+// (16x16x32: A[s][8q+j], B[8q+j][s], j = 0..7; D as before).  Both layouts
+// were checked against a host product on the GPU before use.
 // Every layer is computed transposed -- hidden units along M, samples along
 // N -- so that a layer's D fragment is the next layer's B fragment as is:
 //   H1^T = W1 X^T,  H2^T = W2 A1^T,  z = w3 . A2 + b3.
@@ -112,19 +115,43 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_hw __attribute__((ext_vector_type(8)));
+
+// Four FP32 values to BF16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ bf16x4 pack_bf16x4(float a, float b, float c, float d)
+{
+    const f32x2 lo = {a, b}, hi = {c, d};
+    const bf16x2_hw l = __builtin_convertvector(lo, bf16x2_hw), h = __builtin_convertvector(hi, bf16x2_hw);
+    bf16x4 r;
+    __builtin_memcpy(&r, &l, 4);
+    __builtin_memcpy((char*)&r + 4, &h, 4);
+    return r;
+}
+
 // z of this lane's sample (normalised features xn).  Must be called by all
 // 64 lanes of the wave in uniform control flow; lanes without a sample pass
-// zeros and ignore the result.
+// zeros and ignore the result.  Layer 2 contracts 32 hidden units per
+// v_mfma_f32_16x16x32_bf16 in the order 4q..4q+3, 16+4q..16+4q+3 of each
+// 32-block: that is the order in which a lane already holds the two
+// layer-1 output fragments, so the fragments feed the instruction unchanged
+// and only the weight fragment is read in that order.
 __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, float b3)
 {
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
     const int q = lane >> 4, s = lane & 15;
     uint16_t* xs = L.xs[wave];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) xs[lane * 16 + k] = k < NFEAT ? (uint16_t)f2bf(xn[k]) : (uint16_t)0;
+    for (int k = 0; k < 16; k += 4) {
+        const bf16x4 v = pack_bf16x4(k < NFEAT ? xn[k] : 0.0f, k + 1 < NFEAT ? xn[k + 1] : 0.0f,
+                                     k + 2 < NFEAT ? xn[k + 2] : 0.0f, k + 3 < NFEAT ? xn[k + 3] : 0.0f);
+        *(bf16x4*)&xs[lane * 16 + k] = v;
+    }
     wave_lds_sync();
     float zme = 0.0f;
-#pragma unroll
+#pragma unroll 1
     for (int t = 0; t < 4; ++t) {
         const bf16x4 xb = *(const bf16x4*)&xs[(16 * t + s) * 16 + 4 * q];
         bf16x4 a1[4];
@@ -135,8 +162,7 @@ __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, fl
 #pragma unroll
             for (int j = 0; j < 4; ++j) c[j] = L.b1[hb * 16 + 4 * q + j];
             c = mfma_bf16(w, xb, c);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) a1[hb][j] = f2bf(silu_hw(c[j]));
+            a1[hb] = pack_bf16x4(silu_hw(c[0]), silu_hw(c[1]), silu_hw(c[2]), silu_hw(c[3]));
         }
         float zp = 0.0f;
 #pragma unroll
@@ -145,9 +171,14 @@ __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, fl
 #pragma unroll
             for (int j = 0; j < 4; ++j) c[j] = L.b2[mb * 16 + 4 * q + j];
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) {
-                const bf16x4 w = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 16 + 4 * q];
-                c = mfma_bf16(w, a1[kb], c);
+            for (int kb = 0; kb < 2; ++kb) {
+                const bf16x4 w0 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 4 * q];
+                const bf16x4 w1 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 16 + 4 * q];
+                const s16x8 wa = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
+                const s16x8 xa = {a1[2 * kb][0],     a1[2 * kb][1],     a1[2 * kb][2],     a1[2 * kb][3],
+                                  a1[2 * kb + 1][0], a1[2 * kb + 1][1], a1[2 * kb + 1][2], a1[2 * kb + 1][3]};
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_hw, wa),
+                                                            __builtin_bit_cast(bf16x8_hw, xa), c, 0, 0, 0);
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) zp += L.w3[mb * 16 + 4 * q + j] * silu_hw(c[j]);

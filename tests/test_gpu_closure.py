@@ -191,3 +191,24 @@ def test_box_filter_matches_host_average(gpu):
     c.init_equilibrium(None, np.full(flags.shape, 0.03), None, None)
     c.step(1)
     assert np.allclose(c.filter(4)[1], 0.03, rtol=1e-12)
+
+
+def test_refresh_holds_the_network_coefficient(gpu):
+    """With a refresh interval of 3 the network's coefficients from the
+    evaluating step are held for the next two steps, then re-evaluated."""
+    p, _, _, _ = _problem(16, 17)
+    lat, init = _turbulent_lattice(24, abi.CLOSURE_MLP)
+    lat.set_closure(p, _open_norm())
+    lat.set_closure_refresh(3)
+    lat.init_equilibrium(*init)
+    lat.step(3)                         # one full cycle; the next step evaluates
+    _, _, c_fresh = lat.closure_features()
+    lat.step(1)                         # evaluates the network on this state and holds it
+    _, _, c_held1 = lat.closure_features()
+    lat.step(1)
+    _, _, c_held2 = lat.closure_features()
+    assert np.array_equal(c_held1, c_fresh) and np.array_equal(c_held2, c_fresh)
+    lat.step(1)                         # cycle complete: the next step re-evaluates
+    _, _, c_new = lat.closure_features()
+    assert not np.array_equal(c_new, c_fresh)
+    assert np.all(np.isfinite(lat.macroscopic()[1]))
