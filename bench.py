@@ -61,12 +61,12 @@ def closure_result():
         return None
     with open(path) as fh:
         d = json.load(fh)
-    e = d["eval"]
+    runs = {k: v for k, v in d["eval"].items() if isinstance(v, dict)}
     return {"source": os.path.relpath(path, ROOT),
-            "dissipation_rel_err": {"smagorinsky": round(e["smagorinsky"]["dissipation_rel_err"], 4),
-                                    "mlp": round(e["mlp"]["dissipation_rel_err"], 4)},
-            "spectrum_log_rms": {"smagorinsky": round(e["smagorinsky"]["spectrum_log_rms"], 3),
-                                 "mlp": round(e["mlp"]["spectrum_log_rms"], 3)}}
+            "dissipation_rel_err": {k: round(v["dissipation_rel_err"], 4) for k, v in runs.items()},
+            "spectrum_log_rms": {k: round(v["spectrum_log_rms"], 3) for k, v in runs.items()},
+            "step_cost_vs_smagorinsky": {k: round(v["ms_per_step"] / runs["smagorinsky"]["ms_per_step"] - 1, 4)
+                                         for k, v in runs.items() if k != "smagorinsky"}}
 
 
 def log(msg):

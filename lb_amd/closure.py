@@ -169,9 +169,10 @@ def training_set(filtered, tau0):
     return X, y, w
 
 
-def fit_normalisation(X, q=(0.0005, 0.9995)):
+def fit_normalisation(X, q=(0.0, 1.0)):
     """norm = [mu, sd, lo, hi] (4 x 12): standardisation and the training
-    envelope as quantiles of the standardised features."""
+    envelope as quantiles of the standardised features (default: the full
+    range the network was trained on)."""
     mu = X.mean(axis=0)
     sd = X.std(axis=0)
     sd = np.where(sd > 1e-6, sd, 1.0)
@@ -266,7 +267,7 @@ def dissipation(t, K):
 
 def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_seed=20240601, nu_fine=4e-3,
                    train_steps=3000, train_snaps=(1000, 2000, 3000), eval_steps=4000, eval_every=100, batch=32768,
-                   adam_steps=3000, lr=1e-3, c_les=0.17, seed=7, refresh=10, extra_envelopes=((0.0, 1.0),),
+                   adam_steps=3000, lr=1e-3, c_les=0.17, seed=7, refresh=10, extra_envelopes=((0.0005, 0.9995),),
                    out_json=None):
     """The learned-closure experiment: data from a fine run filtered by 4,
     training, then LES at fine_n/2 against the fine run filtered by 2."""
