@@ -37,8 +37,7 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20240601
 TAU0 = 0.5015          # nu0 = 5e-4
 C_LES = 0.17
-STORAGES = {"f64": abi.STORAGE_F64, "f32": abi.STORAGE_F32, "f16dev": abi.STORAGE_F16DEV,
-            "bf16dev": abi.STORAGE_BF16DEV}
+STORAGES = {v: k for k, v in abi.STORAGE_NAMES.items()}
 
 
 def profiled_traffic(storage_name, n):
@@ -197,7 +196,7 @@ def main():
     variants = {}
     if not args.no_variants:
         vsteps = max(200, args.steps // 5)
-        for name in ("f16dev", "bf16dev", "f64"):
+        for name in ("f16dev", "f16dev_sr", "bf16dev", "bf16dev_sr", "f64"):
             if name == args.storage:
                 continue
             r = run_gpu(n, STORAGES[name], fields, vsteps, 20)
@@ -205,7 +204,8 @@ def main():
             ach = cells * 54 * sb / (r["kernel_ms"] * 1e-3) / 1e9
             variants[name] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
                               "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
-                              "steps": vsteps, "bytes_per_cell": 54 * sb}
+                              "steps": vsteps, "bytes_per_cell": 54 * sb,
+                              "vs_f32_mlups": round(r["mlups"] / res["mlups"], 3)}
             log(f"{name}: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
         # learned closure fused into the same step (FP32 storage), network
         # evaluated every step and every 10 steps (coefficient held between)

@@ -50,8 +50,19 @@ enum { LBM_COLLISION_SRT = 0, LBM_COLLISION_CUM2015 = 1 };
 enum { LBM_CLOSURE_NONE = 0, LBM_CLOSURE_SMAGORINSKY = 1, LBM_CLOSURE_MLP = 2 };
 
 /* Population storage in HBM.  F16DEV/BF16DEV store g_i = f_i - w_i*rho0 and do
- * all collision arithmetic in FP32. */
-enum { LBM_STORAGE_F64 = 0, LBM_STORAGE_F32 = 1, LBM_STORAGE_F16DEV = 2, LBM_STORAGE_BF16DEV = 3 };
+ * all collision arithmetic in FP32, rounding g to nearest.  The *_SR formats
+ * have the same layout; the collide-stream step stores g with stochastic
+ * rounding (unbiased: E[stored] = g; random bits from a hash of the cell and
+ * the step counter, so a run is reproducible).  Other writers (initialisation,
+ * lbm_set_populations, boundary patches) round to nearest. */
+enum {
+    LBM_STORAGE_F64 = 0,
+    LBM_STORAGE_F32 = 1,
+    LBM_STORAGE_F16DEV = 2,
+    LBM_STORAGE_BF16DEV = 3,
+    LBM_STORAGE_F16DEV_SR = 4,
+    LBM_STORAGE_BF16DEV_SR = 5
+};
 
 /* Collision arithmetic.  LITERAL follows the reference's operation order
  * (three-pass central-moment transform, full cumulant set); FAST uses the

@@ -204,6 +204,17 @@ static lbm_status launch_patches(lbm_ctx* c, int odd_next)
     return LBM_OK;
 }
 
+// Seed of the stochastic-rounding stores of step t (any fixed bijective mix
+// of the step counter: runs with the same inputs give the same results).
+static uint32_t step_seed(uint64_t t)
+{
+    uint64_t x = t * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    return (uint32_t)(x >> 32);
+}
+
 template <class ST, int COLL, bool LIT>
 static lbm_status launch_steps(lbm_ctx* c, int nsteps)
 {
@@ -240,6 +251,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     for (int s = 0; s < nsteps; ++s) {
         a.first = c->first ? 1 : 0;
         a.diag = (s == 0 && c->diag_next) ? 1 : 0;
+        if constexpr (ST::sr) a.seed = step_seed((uint64_t)c->steps);
         const bool odd = (c->steps & 1) != 0;
         if constexpr (sizeof(R) == 4 && !LIT) {
             if (mlp && c->cl_count % c->cl_every == 0) {
@@ -309,8 +321,12 @@ static lbm_status with_storage(const lbm_ctx* c, F&& fn)
     switch (c->d.storage) {
     case LBM_STORAGE_F64: return fn(StoreF64{});
     case LBM_STORAGE_F32: return fn(StoreF32{});
-    case LBM_STORAGE_F16DEV: return fn(StoreF16Dev{});
-    case LBM_STORAGE_BF16DEV: return fn(StoreBF16Dev{});
+    // the stochastic-rounding formats share the layout; only the step's
+    // streaming stores round differently
+    case LBM_STORAGE_F16DEV:
+    case LBM_STORAGE_F16DEV_SR: return fn(StoreF16Dev{});
+    case LBM_STORAGE_BF16DEV:
+    case LBM_STORAGE_BF16DEV_SR: return fn(StoreBF16Dev{});
     default: return LBM_E_ARG;
     }
 }
@@ -676,7 +692,7 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     // the learned closure runs in the fast FP32 arithmetic
     if (d->closure == LBM_CLOSURE_MLP && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->closure == LBM_CLOSURE_MLP && d->storage == LBM_STORAGE_F64) return LBM_E_UNSUPPORTED;
-    if (d->storage < LBM_STORAGE_F64 || d->storage > LBM_STORAGE_BF16DEV) return LBM_E_ARG;
+    if (d->storage < LBM_STORAGE_F64 || d->storage > LBM_STORAGE_BF16DEV_SR) return LBM_E_ARG;
     if (d->arith != LBM_ARITH_LITERAL && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->arith == LBM_ARITH_LITERAL && d->storage >= LBM_STORAGE_F16DEV) return LBM_E_ARG;
     // tau > 1/2 keeps omega in (0, 2); NonDim...h:79 asserts this (compiled out by -DNDEBUG)
@@ -859,6 +875,8 @@ lbm_status lbm_step(lbm_ctx* c, int32_t nsteps)
     case LBM_STORAGE_F32: st = launch_by_coll<StoreF32>(c, nsteps); break;
     case LBM_STORAGE_F16DEV: st = launch_by_coll_fast<StoreF16Dev>(c, nsteps); break;
     case LBM_STORAGE_BF16DEV: st = launch_by_coll_fast<StoreBF16Dev>(c, nsteps); break;
+    case LBM_STORAGE_F16DEV_SR: st = launch_by_coll_fast<StoreF16DevSR>(c, nsteps); break;
+    case LBM_STORAGE_BF16DEV_SR: st = launch_by_coll_fast<StoreBF16DevSR>(c, nsteps); break;
     default: return LBM_E_ARG;
     }
     if (st != LBM_OK) return st;

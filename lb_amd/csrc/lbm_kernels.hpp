@@ -102,6 +102,7 @@ struct StepArgs {
     int first;
     int diag;
     int closure;
+    uint32_t seed;                 // per-step seed of the stochastic-rounding stores
 };
 
 // Arguments of the collide-stream step with the learned closure.
@@ -205,26 +206,27 @@ __device__ __forceinline__ void load_pop(typename ST::T* const* p, const typenam
 }
 
 // Streaming write of post-collision population q (goes to x + c_q).
+// h: the cell's random bits for stochastic-rounding storage (else unused).
 template <class ST, bool ODD>
 __device__ __forceinline__ void store_out(typename ST::T* const* p, const typename ST::R* rest, const Cell& c, int q,
-                                          typename ST::R v)
+                                          typename ST::R v, uint32_t h)
 {
     if (ODD)
-        ST::store(p[q], nb(c, q), v, rest[WCLASS[q]]);
+        ST::store(p[q], nb(c, q), v, rest[WCLASS[q]], sr_bits(h, q));
     else
-        ST::store(p[OPP[q]], c.i, v, rest[WCLASS[q]]);
+        ST::store(p[OPP[q]], c.i, v, rest[WCLASS[q]], sr_bits(h, q));
 }
 
 // Write the value cell x will read as incoming population q next step
 // (used when x - c_q is solid and nobody else writes that slot).
 template <class ST, bool ODD>
 __device__ __forceinline__ void store_in_next(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
-                                              int q, typename ST::R v)
+                                              int q, typename ST::R v, uint32_t h)
 {
     if (ODD)
-        ST::store(p[q], c.i, v, rest[WCLASS[q]]);
+        ST::store(p[q], c.i, v, rest[WCLASS[q]], sr_bits(h, 27 + q));
     else
-        ST::store(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]]);
+        ST::store(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]], sr_bits(h, 27 + q));
 }
 
 // Moving-wall update of an inlet cell's own state
@@ -267,9 +269,10 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
                                             typename ST::R rho, typename ST::R ux)
 {
     using R = typename ST::R;
+    const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
     if (!FLAGS || cls == CL_FLUID) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+        for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
     } else {
         const uint64_t nw = a.node[c.i];
         const uint32_t S = (uint32_t)(nw & MASK27);
@@ -279,7 +282,7 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
             // stream the pre-collision state (StreamingAB.h:62-65)
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
+                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q], hsr);
             if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                 R st[Q];
 #pragma unroll
@@ -291,7 +294,7 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 if ((S >> q) & 1u) {
@@ -302,7 +305,7 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
                         v = a.crev20 ? P[21] : P[19];
                     else
                         v = P[OPP[q]];
-                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
+                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
                 }
             }
             // data the boundary patch kernels of this step read
@@ -401,9 +404,10 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             }
         }
 
+        const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
         if (!FLAGS || cls == CL_FLUID) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+            for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
         } else {
             const uint64_t nw = a.node[c.i];
             const uint32_t S = (uint32_t)(nw & MASK27);
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 // stream the pre-collision state (StreamingAB.h:62-65)
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q]);
+                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q], hsr);
                 if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                     R st[Q];
 #pragma unroll
@@ -425,7 +429,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q]);
+                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     if ((S >> q) & 1u) {
@@ -436,7 +440,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                             v = a.crev20 ? P[21] : P[19];
                         else
                             v = P[OPP[q]];
-                        store_in_next<ST, ODD>(a.p, a.rest, c, q, v);
+                        store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
                         if (wall && a.diag) {
                             // momentum the fluid loses across this link: the
                             // outgoing f*_{opp q} leaves, the bounced v enters

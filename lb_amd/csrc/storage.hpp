@@ -3,6 +3,11 @@
 // FP64 and FP32 store f_i.  F16DEV / BF16DEV store the deviation
 // g_i = f_i - w_i*rho0 (well inside the 16-bit formats' range for the
 // low-Mach flows this core targets) and do all arithmetic in FP32.
+//
+// Every policy has store(p, i, v, rest) (round to nearest even) and
+// store(p, i, v, rest, rnd).  The second form rounds stochastically in the
+// *SR policies and to nearest in the others; the collide-stream kernel uses
+// it for the populations it streams, all other writers use the first form.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,24 +19,30 @@ struct StoreF64 {
     using T = double;   // element in HBM
     using R = double;   // arithmetic type
     static constexpr bool dev = false;
+    static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return p[i]; }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { p[i] = v; }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { p[i] = v; }
 };
 
 struct StoreF32 {
     using T = float;
     using R = float;
     static constexpr bool dev = false;
+    static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return p[i]; }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { p[i] = v; }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { p[i] = v; }
 };
 
 struct StoreF16Dev {
     using T = _Float16;
     using R = float;
     static constexpr bool dev = true;
+    static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest) { return (float)p[i] + rest; }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { p[i] = (_Float16)(v - rest); }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
 };
 
 // bfloat16 with round-to-nearest-even on store.
@@ -39,6 +50,7 @@ struct StoreBF16Dev {
     using T = uint16_t;
     using R = float;
     static constexpr bool dev = true;
+    static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest)
     {
         return __uint_as_float((uint32_t)p[i] << 16) + rest;
@@ -48,6 +60,59 @@ struct StoreBF16Dev {
         const uint32_t b = __float_as_uint(v - rest);
         p[i] = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
     }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
 };
+
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Stochastic rounding (gfx950 v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32): the
+// stored value is one of the two neighbours of g in the 16-bit format and is
+// the upper one with probability (g - lower) / (upper - lower), so that
+// E[stored] = g.  A rounding error that repeats from step to step on a slowly
+// changing field then averages out instead of adding up.  For results in the
+// normal range the conversion equals truncation of the FP32 bit pattern
+// after adding the top 13 (FP16) or 16 (BF16) bits of rnd to the bits it
+// drops.  Same layout in HBM as the round-to-nearest policies.
+struct StoreF16DevSR : StoreF16Dev {
+    static constexpr bool sr = true;
+    using StoreF16Dev::store;
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
+    {
+        f16x2_t h = {(_Float16)0, (_Float16)0};
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, v - rest, rnd, false);
+        p[i] = h[0];
+    }
+};
+
+struct StoreBF16DevSR : StoreBF16Dev {
+    static constexpr bool sr = true;
+    using StoreBF16Dev::store;
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
+    {
+        bf16x2_t h = {(__bf16)0.0f, (__bf16)0.0f};
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v - rest, rnd, false);
+        uint16_t b;
+        __builtin_memcpy(&b, &h, 2);
+        p[i] = b;
+    }
+};
+
+// Random bits of one cell and step for the stochastic-rounding stores: the
+// "lowbias32" integer hash (C. Wellons, hash-prospector; xor-shift-multiply
+// rounds) of the cell index xor the step seed; population q uses
+// h + q * 0x9E3779B9 (a Weyl sequence, whose high bits the conversions read).
+__device__ __forceinline__ uint32_t sr_hash(uint32_t i, uint32_t seed)
+{
+    uint32_t x = i ^ seed;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t sr_bits(uint32_t h, int q) { return h + (uint32_t)q * 0x9E3779B9u; }
 
 }  // namespace lbm

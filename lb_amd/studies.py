@@ -1,7 +1,8 @@
 """Measured studies on the GPU (host orchestration over the C ABI).
 
 c3  Taylor-Green vortex 384^3, Re 1600, cumulant: FP64 storage as the
-    reference run against FP32, FP16-deviation and BF16-deviation storage
+    reference run against FP32, FP16-deviation and BF16-deviation storage,
+    each 16-bit format with round-to-nearest and with stochastic rounding
     (all FP32 arithmetic except FP64); rel-L2 of u and rho and the
     kinetic-energy error over 10 000 steps, MLUPS per storage.
 c4  flow past a voxelized obstacle, 512 x 256 x 256: synthetic OFF scene,
@@ -35,14 +36,14 @@ def _rel(a, b):
     return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b * b)))
 
 
-def c3(n=384, steps=10000, every=1000, u0=0.05, storages=("f64", "f32", "f16dev", "bf16dev")):
+def c3(n=384, steps=10000, every=1000, u0=0.05,
+       storages=("f64", "f32", "f16dev", "bf16dev", "f16dev_sr", "bf16dev_sr")):
     L = n / (2.0 * np.pi)
     nu = u0 * L / 1600.0
     tau = 3.0 * nu + 0.5
     rho0, ux, uy, uz = taylor_green(n, u0)
     flags = np.ones((n, n, n), np.int32)
-    codes = {"f64": abi.STORAGE_F64, "f32": abi.STORAGE_F32, "f16dev": abi.STORAGE_F16DEV,
-             "bf16dev": abi.STORAGE_BF16DEV}
+    codes = {v: k for k, v in abi.STORAGE_NAMES.items()}
     lats = {}
     for s in storages:
         lats[s] = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, storage=codes[s], arith=abi.ARITH_FAST,
@@ -170,10 +171,13 @@ if __name__ == "__main__":
     ap.add_argument("study", choices=["c3", "c4"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--n", type=int, default=384)
+    ap.add_argument("--every", type=int, default=1000)
+    ap.add_argument("--storages", default=None, help="comma-separated, c3 only (default: all)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.study == "c3":
-        r = c3(n=a.n, steps=a.steps or 10000)
+        kw = {"storages": tuple(a.storages.split(","))} if a.storages else {}
+        r = c3(n=a.n, steps=a.steps or 10000, every=a.every, **kw)
     else:
         r = c4(steps=a.steps or 5000)
     if a.out:

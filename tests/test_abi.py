@@ -80,8 +80,10 @@ def test_validation_rejects_bad_descriptors():
     bad = list(bc) + [abi.BcNode(0, 0, 0, abi.BC_WALL)]
     assert _create_status(make_desc(flags, bad, tau=0.6)) == abi.LBM_E_ARG
     # literal arithmetic is FP64/FP32 only
-    assert _create_status(make_desc(flags, bc, tau=0.6, storage=abi.STORAGE_F16DEV,
-                                    arith=abi.ARITH_LITERAL)) == abi.LBM_E_ARG
+    for st in (abi.STORAGE_F16DEV, abi.STORAGE_BF16DEV, abi.STORAGE_F16DEV_SR, abi.STORAGE_BF16DEV_SR):
+        assert _create_status(make_desc(flags, bc, tau=0.6, storage=st, arith=abi.ARITH_LITERAL)) == abi.LBM_E_ARG
+    # unknown storage code
+    assert _create_status(make_desc(flags, bc, tau=0.6, storage=6, arith=abi.ARITH_FAST)) == abi.LBM_E_ARG
 
 
 def test_valid_descriptor_without_device_reports_hip_error():

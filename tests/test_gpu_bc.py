@@ -108,8 +108,9 @@ def test_channel_chunked_steps_match(gpu):
 # near-equilibrium start is rounding noise (~1e-17), so the two operation
 # orders differ by ~1e-9 there; the bound is set an order above that.
 @pytest.mark.parametrize("storage,tol", [(abi.STORAGE_F64, 1e-8), (abi.STORAGE_F32, 2e-5),
-                                         (abi.STORAGE_F16DEV, 2e-3), (abi.STORAGE_BF16DEV, 1e-2)],
-                         ids=["f64", "f32", "f16dev", "bf16dev"])
+                                         (abi.STORAGE_F16DEV, 2e-3), (abi.STORAGE_BF16DEV, 1e-2),
+                                         (abi.STORAGE_F16DEV_SR, 2e-3), (abi.STORAGE_BF16DEV_SR, 1e-2)],
+                         ids=["f64", "f32", "f16dev", "bf16dev", "f16dev_sr", "bf16dev_sr"])
 def test_channel_fast_vs_fp64_oracle(gpu, storage, tol):
     flags, bc = _channel()
     g, o = _run_pair(flags, bc, 100, _init(flags, seed=9), collision=abi.COLLISION_CUM2015,

@@ -116,7 +116,9 @@ def test_cavity_fast_fp64(gpu):
 
 
 @pytest.mark.parametrize("storage,tol", [(abi.STORAGE_F32, 1e-4), (abi.STORAGE_F16DEV, 5e-3),
-                                         (abi.STORAGE_BF16DEV, 5e-2)], ids=["f32", "f16dev", "bf16dev"])
+                                         (abi.STORAGE_BF16DEV, 5e-2), (abi.STORAGE_F16DEV_SR, 5e-3),
+                                         (abi.STORAGE_BF16DEV_SR, 5e-2)],
+                         ids=["f32", "f16dev", "bf16dev", "f16dev_sr", "bf16dev_sr"])
 def test_periodic_reduced_precision_vs_fp64(gpu, storage, tol):
     shape = (16, 16, 16)
     flags = np.ones(shape, np.int32)
