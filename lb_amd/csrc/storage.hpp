@@ -15,14 +15,28 @@
 
 namespace lbm {
 
+// Element i of a population plane.  The byte offset is formed in 32 bits (a
+// plane holds at most 4 GiB, checked when the lattice is created), so the
+// access is a 64-bit uniform base plus a 32-bit per-lane offset.
+template <class T>
+__device__ __forceinline__ const T& at(const T* p, uint32_t i)
+{
+    return *(const T*)((const char*)p + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ T& at(T* p, uint32_t i)
+{
+    return *(T*)((char*)p + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+
 struct StoreF64 {
     using T = double;   // element in HBM
     using R = double;   // arithmetic type
     static constexpr bool dev = false;
     static constexpr bool sr = false;
-    __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return p[i]; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { p[i] = v; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { p[i] = v; }
+    __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { at(p, i) = v; }
 };
 
 struct StoreF32 {
@@ -30,9 +44,9 @@ struct StoreF32 {
     using R = float;
     static constexpr bool dev = false;
     static constexpr bool sr = false;
-    __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return p[i]; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { p[i] = v; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { p[i] = v; }
+    __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { at(p, i) = v; }
 };
 
 struct StoreF16Dev {
@@ -40,8 +54,8 @@ struct StoreF16Dev {
     using R = float;
     static constexpr bool dev = true;
     static constexpr bool sr = false;
-    __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest) { return (float)p[i] + rest; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { p[i] = (_Float16)(v - rest); }
+    __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest) { return (float)at(p, i) + rest; }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = (_Float16)(v - rest); }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
 };
 
@@ -53,12 +67,12 @@ struct StoreBF16Dev {
     static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest)
     {
-        return __uint_as_float((uint32_t)p[i] << 16) + rest;
+        return __uint_as_float((uint32_t)at(p, i) << 16) + rest;
     }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest)
     {
         const uint32_t b = __float_as_uint(v - rest);
-        p[i] = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+        at(p, i) = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
     }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
 };
@@ -79,9 +93,11 @@ struct StoreF16DevSR : StoreF16Dev {
     using StoreF16Dev::store;
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
     {
-        f16x2_t h = {(_Float16)0, (_Float16)0};
+        // the conversion writes the low half and keeps the high half of its
+        // first operand; passing the random word there saves a register copy
+        f16x2_t h = __builtin_bit_cast(f16x2_t, rnd);
         h = __builtin_amdgcn_cvt_sr_f16_f32(h, v - rest, rnd, false);
-        p[i] = h[0];
+        at(p, i) = h[0];
     }
 };
 
@@ -90,11 +106,11 @@ struct StoreBF16DevSR : StoreBF16Dev {
     using StoreBF16Dev::store;
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
     {
-        bf16x2_t h = {(__bf16)0.0f, (__bf16)0.0f};
+        bf16x2_t h = __builtin_bit_cast(bf16x2_t, rnd);
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v - rest, rnd, false);
         uint16_t b;
         __builtin_memcpy(&b, &h, 2);
-        p[i] = b;
+        at(p, i) = b;
     }
 };
 
