@@ -9,7 +9,8 @@ c4  flow past a voxelized obstacle, 512 x 256 x 256: synthetic OFF scene,
     the host voxelizer and boundary mesher, equilibrium velocity inlet at
     x = 0, interpolated density outlet at x = 511, index wrap along y and z,
     cumulant + Smagorinsky; drag from the wall-link momentum exchange, FP32
-    against FP64 storage, MLUPS.
+    and FP16-deviation storage (stochastic and nearest rounding) against FP64
+    storage, MLUPS.
 
 usage: python -m lb_amd.studies c3|c4 [--steps N] [--out file.json]
 """
@@ -131,7 +132,9 @@ def c4(steps=5000, every=250, fp64_steps=None):
                        "cumulant-2015 + Smagorinsky C=0.17", "geometry": info}
     fp64_steps = fp64_steps or steps
     ux0 = np.where(flags == 0, 0.0, u_in)
-    for name, st, nst in (("f32", abi.STORAGE_F32, steps), ("f64", abi.STORAGE_F64, fp64_steps)):
+    runs = (("f32", abi.STORAGE_F32, steps), ("f16dev_sr", abi.STORAGE_F16DEV_SR, steps),
+            ("f16dev", abi.STORAGE_F16DEV, steps), ("f64", abi.STORAGE_F64, fp64_steps))
+    for name, st, nst in runs:
         lat = Lattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
                                 storage=st, arith=abi.ARITH_FAST, compat=abi.COMPAT_REFERENCE, periodic=(0, 1, 1),
                                 tau=tau, nu=nu, c_les=0.17))
@@ -154,12 +157,15 @@ def c4(steps=5000, every=250, fp64_steps=None):
                      "mlups": float(info["fluid_cells"] / (np.median(ms) * 1e-3) / 1e6),
                      "mlups_all_cells": float(flags.size / (np.median(ms) * 1e-3) / 1e6)}
         lat.close()
-    n = min(len(res["f32"]["force"]), len(res["f64"]["force"]))
-    f32 = np.array(res["f32"]["force"][:n])
-    f64 = np.array(res["f64"]["force"][:n])
-    half = n // 2
-    res["drag_rel_diff_f32_vs_f64"] = float(np.mean(np.abs(f32[half:, 0] - f64[half:, 0])) /
-                                            np.mean(np.abs(f64[half:, 0])))
+    for name, _, _ in runs[:-1]:
+        n = min(len(res[name]["force"]), len(res["f64"]["force"]))
+        fx = np.array(res[name]["force"][:n])
+        f64 = np.array(res["f64"]["force"][:n])
+        half = n // 2
+        res[f"drag_rel_diff_{name}_vs_f64"] = float(np.mean(np.abs(fx[half:, 0] - f64[half:, 0])) /
+                                                   np.mean(np.abs(f64[half:, 0])))
+    f64 = np.array(res["f64"]["force"])
+    half = len(f64) // 2
     res["drag_coefficient_f64"] = float(np.mean(f64[half:, 0]) / (0.5 * u_in ** 2 * np.pi * (D / 2) ** 2))
     return res
 
