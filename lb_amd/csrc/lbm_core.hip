@@ -1157,11 +1157,13 @@ lbm_status lbm_set_closure(lbm_ctx* c, const float* params, const float* norm)
     if (c->d.closure != LBM_CLOSURE_MLP) return LBM_E_STATE;
     MlpNorm N;
     for (int k = 0; k < NFEAT; ++k) {
+        const float sd = norm[NFEAT + k];
         N.mu[k] = norm[k];
-        N.sd[k] = norm[NFEAT + k];
+        N.isd[k] = 1.0f / sd;
         N.lo[k] = norm[2 * NFEAT + k];
         N.hi[k] = norm[3 * NFEAT + k];
-        if (!(N.sd[k] > 0.0f) || !std::isfinite(N.sd[k]) || !std::isfinite(N.mu[k])) return LBM_E_ARG;
+        if (!(sd > 0.0f) || !std::isfinite(sd) || !std::isfinite(N.mu[k]) || !std::isfinite(N.isd[k]))
+            return LBM_E_ARG;
     }
     for (int i = 0; i < NPARAM; ++i)
         if (!std::isfinite(params[i])) return LBM_E_ARG;

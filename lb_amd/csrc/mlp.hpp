@@ -76,9 +76,10 @@ struct MlpWeights {
     float c;              // head scale (Smagorinsky constant)
 };
 
-// Input normalisation and training envelope (normalised units).
+// Input normalisation and training envelope (normalised units); isd holds
+// the reciprocal standard deviations, so normalising costs no division.
 struct MlpNorm {
-    float mu[NFEAT], sd[NFEAT], lo[NFEAT], hi[NFEAT];
+    float mu[NFEAT], isd[NFEAT], lo[NFEAT], hi[NFEAT];
 };
 
 // LDS image of the weights plus a per-wave feature staging area.
@@ -138,6 +139,10 @@ __device__ __forceinline__ bf16x4 pack_bf16x4(float a, float b, float c, float d
 // 32-block: that is the order in which a lane already holds the two
 // layer-1 output fragments, so the fragments feed the instruction unchanged
 // and only the weight fragment is read in that order.
+// Every weight fragment, bias and output weight is read from LDS once per
+// wave and applied to the four tiles in turn: four independent products per
+// fragment, and a quarter of the LDS traffic of a tile-by-tile loop.  Each
+// tile's arithmetic, and its order, is the same as evaluating it alone.
 __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, float b3)
 {
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
@@ -150,42 +155,59 @@ __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, fl
         *(bf16x4*)&xs[lane * 16 + k] = v;
     }
     wave_lds_sync();
-    float zme = 0.0f;
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-        const bf16x4 xb = *(const bf16x4*)&xs[(16 * t + s) * 16 + 4 * q];
-        bf16x4 a1[4];
+    bf16x4 xb[4];   // inputs of tile t: the cells of lanes 16t .. 16t+15
 #pragma unroll
-        for (int hb = 0; hb < 4; ++hb) {
-            const bf16x4 w = *(const bf16x4*)&L.w1[(hb * 16 + s) * 16 + 4 * q];
-            f32x4 c;
+    for (int t = 0; t < 4; ++t) xb[t] = *(const bf16x4*)&xs[(16 * t + s) * 16 + 4 * q];
+    bf16x4 a1[4][4];   // layer-1 activations [tile][hidden block]
 #pragma unroll
-            for (int j = 0; j < 4; ++j) c[j] = L.b1[hb * 16 + 4 * q + j];
-            c = mfma_bf16(w, xb, c);
-            a1[hb] = pack_bf16x4(silu_hw(c[0]), silu_hw(c[1]), silu_hw(c[2]), silu_hw(c[3]));
+    for (int hb = 0; hb < 4; ++hb) {
+        const bf16x4 w = *(const bf16x4*)&L.w1[(hb * 16 + s) * 16 + 4 * q];
+        f32x4 bias;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bias[j] = L.b1[hb * 16 + 4 * q + j];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const f32x4 c = mfma_bf16(w, xb[t], bias);
+            a1[t][hb] = pack_bf16x4(silu_hw(c[0]), silu_hw(c[1]), silu_hw(c[2]), silu_hw(c[3]));
         }
-        float zp = 0.0f;
+    }
+    float zp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            f32x4 c;
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x4 c[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) c[j] = L.b2[mb * 16 + 4 * q + j];
+        for (int j = 0; j < 4; ++j) c[0][j] = L.b2[mb * 16 + 4 * q + j];
 #pragma unroll
-            for (int kb = 0; kb < 2; ++kb) {
-                const bf16x4 w0 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 4 * q];
-                const bf16x4 w1 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 16 + 4 * q];
-                const s16x8 wa = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
-                const s16x8 xa = {a1[2 * kb][0],     a1[2 * kb][1],     a1[2 * kb][2],     a1[2 * kb][3],
-                                  a1[2 * kb + 1][0], a1[2 * kb + 1][1], a1[2 * kb + 1][2], a1[2 * kb + 1][3]};
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_hw, wa),
-                                                            __builtin_bit_cast(bf16x8_hw, xa), c, 0, 0, 0);
+        for (int t = 1; t < 4; ++t) c[t] = c[0];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            const bf16x4 w0 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 4 * q];
+            const bf16x4 w1 = *(const bf16x4*)&L.w2[(mb * 16 + s) * NHID + kb * 32 + 16 + 4 * q];
+            const s16x8 wa = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3]};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const s16x8 xa = {a1[t][2 * kb][0],     a1[t][2 * kb][1],     a1[t][2 * kb][2],
+                                  a1[t][2 * kb][3],     a1[t][2 * kb + 1][0], a1[t][2 * kb + 1][1],
+                                  a1[t][2 * kb + 1][2], a1[t][2 * kb + 1][3]};
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_hw, wa),
+                                                               __builtin_bit_cast(bf16x8_hw, xa), c[t], 0, 0, 0);
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) zp += L.w3[mb * 16 + 4 * q + j] * silu_hw(c[j]);
         }
-        zp += __shfl_xor(zp, 16, 64);
-        zp += __shfl_xor(zp, 32, 64);
-        if (q == t) zme = zp;
+        float w3[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w3[j] = L.w3[mb * 16 + 4 * q + j];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) zp[t] += w3[j] * silu_hw(c[t][j]);
+    }
+    float zme = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        float v = zp[t];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (q == t) zme = v;
     }
     wave_lds_sync();   // the staging area is reused by the next call
     return zme + b3;
@@ -197,7 +219,7 @@ __device__ __forceinline__ bool mlp_normalise(const MlpNorm& N, const float* x, 
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < NFEAT; ++k) {
-        xn[k] = (x[k] - N.mu[k]) / N.sd[k];
+        xn[k] = (x[k] - N.mu[k]) * N.isd[k];
         ok = ok && xn[k] >= N.lo[k] && xn[k] <= N.hi[k];
     }
     return ok;
