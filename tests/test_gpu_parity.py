@@ -157,6 +157,34 @@ def test_conservation_periodic(gpu):
     assert np.all(np.abs(m1[1:] - m0[1:]) <= 1e-12 * abs(m0[0]))
 
 
+@pytest.mark.parametrize("storage", [abi.STORAGE_F64, abi.STORAGE_F32, abi.STORAGE_F16DEV],
+                         ids=["f64", "f32", "f16dev"])
+def test_conservation_equals_host_sum(gpu, storage):
+    """The device's FP64 mass/momentum reduction equals a brute-force host
+    sum over the cells."""
+    flags = np.ones((12, 10, 14), np.int32)
+    L = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+                          storage=storage, arith=abi.ARITH_FAST, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1),
+                          tau=0.55, c_les=0.17))
+    u = random_velocity_field(flags.shape, 0.03, seed=17)
+    L.init_equilibrium(1.0 + 0.1 * u[2], u[0], u[1], u[2])
+    L.step(7)
+    m = L.conservation()
+    f = L.populations()
+    fluid = flags != 0
+    mass = f.sum()
+    rho, vx, vy, vz = L.macroscopic()
+    mom = np.array([(rho * v)[fluid].sum() for v in (vx, vy, vz)])
+    scale = np.abs(rho[fluid]).sum()
+    print(f"[conservation vs host, storage={storage}] mass {m[0]:.15g} host {mass:.15g}; "
+          f"momentum {m[1:]} host {mom}", flush=True)
+    assert abs(m[0] - mass) <= 1e-12 * mass
+    # momentum: the host forms rho * u from the stored macroscopic fields, which
+    # the kernel rounds in the storage's arithmetic (FP32 below FP64)
+    tol = 1e-13 if storage == abi.STORAGE_F64 else 1e-6
+    assert np.all(np.abs(m[1:] - mom) <= tol * scale)
+
+
 def test_set_get_populations_roundtrip(gpu):
     flags, bc, ux = cavity_setup(6)
     L = Lattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
