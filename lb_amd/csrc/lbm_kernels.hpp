@@ -192,29 +192,38 @@ __device__ __forceinline__ void read_slot(const Cell& c, int q, bool odd, int& p
     }
 }
 
+// A wave's access to population q covers whole cache lines when it is not
+// shifted along x: every access of the even step, and on the odd step the
+// directions with c_x = 0 (tiles are whole rows or power-of-two segments).
+__host__ __device__ constexpr bool line_aligned(bool odd, int q) { return !odd || CX[q] == 0; }
+__host__ __device__ constexpr int read_plane(bool odd, int q) { return odd ? OPP[q] : q; }
+__device__ __forceinline__ uint32_t read_index(const Cell& c, bool odd, int q) { return odd ? nb(c, OPP[q]) : c.i; }
+
 template <class ST, bool ODD>
 __device__ __forceinline__ void load_pop(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
                                          typename ST::R* f)
 {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (ODD)
-            f[q] = ST::load(p[OPP[q]], nb(c, OPP[q]), rest[WCLASS[q]]);
-        else
-            f[q] = ST::load(p[q], c.i, rest[WCLASS[q]]);
-    }
+    for (int q = 0; q < Q; ++q)
+        if (line_aligned(ODD, q))
+            f[q] = ST::template get<true>(p[read_plane(ODD, q)], read_index(c, ODD, q), rest[WCLASS[q]]);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (!line_aligned(ODD, q))
+            f[q] = ST::template get<false>(p[read_plane(ODD, q)], read_index(c, ODD, q), rest[WCLASS[q]]);
 }
 
 // Streaming write of post-collision population q (goes to x + c_q).
-// h: the cell's random bits for stochastic-rounding storage (else unused).
-template <class ST, bool ODD>
+// h: the cell's random bits for stochastic-rounding storage (else unused);
+// NT: the streaming hint, for the line-aligned directions.
+template <class ST, bool ODD, bool NT>
 __device__ __forceinline__ void store_out(typename ST::T* const* p, const typename ST::R* rest, const Cell& c, int q,
                                           typename ST::R v, uint32_t h)
 {
     if (ODD)
-        ST::store(p[q], nb(c, q), v, rest[WCLASS[q]], sr_bits(h, q));
+        ST::template put<NT>(p[q], nb(c, q), v, rest[WCLASS[q]], sr_bits(h, q));
     else
-        ST::store(p[OPP[q]], c.i, v, rest[WCLASS[q]], sr_bits(h, q));
+        ST::template put<NT>(p[OPP[q]], c.i, v, rest[WCLASS[q]], sr_bits(h, q));
 }
 
 // Write the value cell x will read as incoming population q next step
@@ -224,9 +233,9 @@ __device__ __forceinline__ void store_in_next(typename ST::T* const* p, const ty
                                               int q, typename ST::R v, uint32_t h)
 {
     if (ODD)
-        ST::store(p[q], c.i, v, rest[WCLASS[q]], sr_bits(h, 27 + q));
+        ST::template put<false>(p[q], c.i, v, rest[WCLASS[q]], sr_bits(h, 27 + q));
     else
-        ST::store(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]], sr_bits(h, 27 + q));
+        ST::template put<false>(p[OPP[q]], nb(c, OPP[q]), v, rest[WCLASS[q]], sr_bits(h, 27 + q));
 }
 
 // Moving-wall update of an inlet cell's own state
@@ -272,7 +281,11 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
     const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
     if (!FLAGS || cls == CL_FLUID) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
+        for (int q = 0; q < Q; ++q)
+            if (line_aligned(ODD, q)) store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (!line_aligned(ODD, q)) store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
     } else {
         const uint64_t nw = a.node[c.i];
         const uint32_t S = (uint32_t)(nw & MASK27);
@@ -282,7 +295,12 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
             // stream the pre-collision state (StreamingAB.h:62-65)
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q], hsr);
+                if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
+                    store_out<ST, ODD, true>(a.p, a.rest, c, q, f[q], hsr);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
+                    store_out<ST, ODD, false>(a.p, a.rest, c, q, f[q], hsr);
             if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                 R st[Q];
 #pragma unroll
@@ -294,7 +312,12 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
+                if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
+                    store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
+                    store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 if ((S >> q) & 1u) {
@@ -407,7 +430,11 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
         if (!FLAGS || cls == CL_FLUID) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
+            for (int q = 0; q < Q; ++q)
+                if (line_aligned(ODD, q)) store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!line_aligned(ODD, q)) store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
         } else {
             const uint64_t nw = a.node[c.i];
             const uint32_t S = (uint32_t)(nw & MASK27);
@@ -417,7 +444,12 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 // stream the pre-collision state (StreamingAB.h:62-65)
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, f[q], hsr);
+                    if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
+                        store_out<ST, ODD, true>(a.p, a.rest, c, q, f[q], hsr);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
+                        store_out<ST, ODD, false>(a.p, a.rest, c, q, f[q], hsr);
                 if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                     R st[Q];
 #pragma unroll
@@ -429,7 +461,12 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u)) store_out<ST, ODD>(a.p, a.rest, c, q, P[q], hsr);
+                    if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
+                        store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
+                        store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     if ((S >> q) & 1u) {

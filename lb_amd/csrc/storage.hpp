@@ -5,9 +5,11 @@
 // low-Mach flows this core targets) and do all arithmetic in FP32.
 //
 // Every policy has store(p, i, v, rest) (round to nearest even) and
-// store(p, i, v, rest, rnd).  The second form rounds stochastically in the
-// *SR policies and to nearest in the others; the collide-stream kernel uses
-// it for the populations it streams, all other writers use the first form.
+// put<NT>(p, i, v, rest, rnd).  The second form rounds stochastically in
+// the *SR policies and to nearest in the others, and takes the streaming
+// hint NT; the collide-stream kernel uses it for the populations it streams,
+// all other writers use the first form.  load(p, i, rest) and
+// get<NT>(p, i, rest) likewise.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,6 +31,30 @@ __device__ __forceinline__ T& at(T* p, uint32_t i)
     return *(T*)((char*)p + (uint32_t)(i * (uint32_t)sizeof(T)));
 }
 
+// Plain or non-temporal (streaming) access to element i, chosen at compile
+// time (a run-time choice between two accesses to one address lets the
+// compiler merge them and drop the hint).  The collide-stream step streams
+// every population once per step; where a wave's access covers whole cache
+// lines (every access of the even step, and the directions without an x
+// component on the odd step), the streaming hint keeps those lines from
+// displacing the partial lines that neighbouring tiles share.
+template <bool NT, class T>
+__device__ __forceinline__ T ld(const T* p, uint32_t i)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(&at(p, i));
+    else
+        return at(p, i);
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st(T* p, uint32_t i, T v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, &at(p, i));
+    else
+        at(p, i) = v;
+}
+
 struct StoreF64 {
     using T = double;   // element in HBM
     using R = double;   // arithmetic type
@@ -36,7 +62,16 @@ struct StoreF64 {
     static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { at(p, i) = v; }
+    template <bool NT>
+    __device__ __forceinline__ static R get(const T* p, uint32_t i, R)
+    {
+        return ld<NT>(p, i);
+    }
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R, uint32_t)
+    {
+        st<NT>(p, i, v);
+    }
 };
 
 struct StoreF32 {
@@ -46,7 +81,16 @@ struct StoreF32 {
     static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R, uint32_t) { at(p, i) = v; }
+    template <bool NT>
+    __device__ __forceinline__ static R get(const T* p, uint32_t i, R)
+    {
+        return ld<NT>(p, i);
+    }
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R, uint32_t)
+    {
+        st<NT>(p, i, v);
+    }
 };
 
 struct StoreF16Dev {
@@ -56,7 +100,16 @@ struct StoreF16Dev {
     static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest) { return (float)at(p, i) + rest; }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = (_Float16)(v - rest); }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
+    template <bool NT>
+    __device__ __forceinline__ static R get(const T* p, uint32_t i, R rest)
+    {
+        return (float)ld<NT>(p, i) + rest;
+    }
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R rest, uint32_t)
+    {
+        st<NT>(p, i, (_Float16)(v - rest));
+    }
 };
 
 // bfloat16 with round-to-nearest-even on store.
@@ -69,12 +122,22 @@ struct StoreBF16Dev {
     {
         return __uint_as_float((uint32_t)at(p, i) << 16) + rest;
     }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest)
+    template <bool NT>
+    __device__ __forceinline__ static R get(const T* p, uint32_t i, R rest)
+    {
+        return __uint_as_float((uint32_t)ld<NT>(p, i) << 16) + rest;
+    }
+    __device__ __forceinline__ static uint16_t rne(R v, R rest)
     {
         const uint32_t b = __float_as_uint(v - rest);
-        at(p, i) = (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+        return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
     }
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t) { store(p, i, v, rest); }
+    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = rne(v, rest); }
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R rest, uint32_t)
+    {
+        st<NT>(p, i, rne(v, rest));
+    }
 };
 
 typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
@@ -90,27 +153,27 @@ typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 // drops.  Same layout in HBM as the round-to-nearest policies.
 struct StoreF16DevSR : StoreF16Dev {
     static constexpr bool sr = true;
-    using StoreF16Dev::store;
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R rest, uint32_t rnd)
     {
         // the conversion writes the low half and keeps the high half of its
         // first operand; passing the random word there saves a register copy
         f16x2_t h = __builtin_bit_cast(f16x2_t, rnd);
         h = __builtin_amdgcn_cvt_sr_f16_f32(h, v - rest, rnd, false);
-        at(p, i) = h[0];
+        st<NT>(p, i, (_Float16)h[0]);
     }
 };
 
 struct StoreBF16DevSR : StoreBF16Dev {
     static constexpr bool sr = true;
-    using StoreBF16Dev::store;
-    __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest, uint32_t rnd)
+    template <bool NT>
+    __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R rest, uint32_t rnd)
     {
         bf16x2_t h = __builtin_bit_cast(bf16x2_t, rnd);
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v - rest, rnd, false);
         uint16_t b;
         __builtin_memcpy(&b, &h, 2);
-        at(p, i) = b;
+        st<NT>(p, i, b);
     }
 };
 
