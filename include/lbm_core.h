@@ -120,7 +120,7 @@ typedef struct lbm_desc {
  * boundary lists.  Neighbours outside the array on an axis without index wrap
  * are treated as the reference's out-of-array reads are restated: populations
  * streaming from there are left untouched, streaming there is dropped (the
- * core pads a one-cell solid halo for their slots; StreamingAB.h:51-56 has no
+ * core pads a solid halo for their slots; StreamingAB.h:51-56 has no
  * bounds checks).  A wall-list node with such a neighbour is
  * LBM_E_UNSUPPORTED, as are flags below -3, outlet or ghost entries on inlet
  * cells, and inlet/outlet rules that would read an inlet cell as a neighbour. */

@@ -709,15 +709,19 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     L.w[0] = d->periodic[0] != 0;
     L.w[1] = d->periodic[1] != 0;
     L.w[2] = d->periodic[2] != 0;
-    L.px = L.w[0] ? 0 : 1;
+    const size_t elem = d->storage == LBM_STORAGE_F64 ? 8 : d->storage == LBM_STORAGE_F32 ? 4 : 2;
+    const int line = (int)(128 / elem);   // populations per 128-byte cache line
+    // A halo along x is a whole line of solid cells on each side, so that the
+    // caller's rows start on a line and the step's line-aligned accesses are
+    // aligned in fact; only its innermost cells are ever read.
+    L.px = L.w[0] ? 0 : line;
     L.py = L.w[1] ? 0 : 1;
     L.pz = L.w[2] ? 0 : 1;
-    L.nx = L.ux + 2 * L.px;
+    L.nx = L.w[0] ? L.ux : (L.px + L.ux + 1 + line - 1) / line * line;
     L.ny = L.uy + 2 * L.py;
     L.nz = L.uz + 2 * L.pz;
     const size_t N = (size_t)L.nx * L.ny * L.nz;
     const size_t NU = (size_t)L.ux * L.uy * L.uz;
-    const size_t elem = d->storage == LBM_STORAGE_F64 ? 8 : d->storage == LBM_STORAGE_F32 ? 4 : 2;
     if (N * elem > 0xFFFFFFFFull || N >= 0x7FFFFFFFull) return LBM_E_ARG;  // 32-bit plane offsets
 
     Plan P;
@@ -753,7 +757,7 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
         return LBM_E_HIP;
     }
     // tile geometry: 256 threads = (1 << bx_log2) x by cells
-    const int bxl = pick_bx_log2(L.nx);
+    const int bxl = pick_bx_log2(L.ux);
     c->g.nx = L.nx;
     c->g.ny = L.ny;
     c->g.nz = L.nz;
@@ -765,9 +769,9 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     c->g.uz = L.uz;
     c->g.bx_log2 = bxl;
     c->g.by = TPB >> bxl;
-    c->g.ntx = (L.nx + (1 << bxl) - 1) >> bxl;
-    c->g.nty = (L.ny + c->g.by - 1) / c->g.by;
-    c->g.ntiles = c->g.ntx * c->g.nty * L.nz;
+    c->g.ntx = (L.ux + (1 << bxl) - 1) >> bxl;
+    c->g.nty = (L.uy + c->g.by - 1) / c->g.by;
+    c->g.ntiles = c->g.ntx * c->g.nty * L.uz;
     c->g.wx = L.w[0];
     c->g.wy = L.w[1];
     c->g.wz = L.w[2];

@@ -10,7 +10,8 @@
 // which reads and writes two full lattices plus rho/u/omega every step.
 //
 // A-A layout: one population array P[q][cell] (structure of arrays, x
-// fastest, one-cell solid halo on every non-periodic axis).  With t the
+// fastest, a solid halo on every non-periodic axis: one cell in y and z, a
+// whole cache line of cells in x so that rows start on a line).  With t the
 // number of completed steps:
 //   even t: cell x reads f_q from P[q][x] and writes post-collision f*_q to
 //           P[opp q][x];
@@ -58,7 +59,8 @@ constexpr uint64_t MASK27 = (1ull << 27) - 1;
 
 struct Geo {
     int nx, ny, nz;     // padded lattice
-    int px, py, pz;     // halo width per axis (0 on index-wrap axes)
+    int px, py, pz;     // offset of the caller's lattice: halo width per axis (0 on
+                        // index-wrap axes; along x a whole cache line of solid cells)
     int ux, uy, uz;     // caller's lattice
     int bx_log2, by;    // tile: (1 << bx_log2) x by cells, 256 threads
     int ntx, nty;       // tiles along x and y
@@ -113,16 +115,6 @@ struct MlpStepArgs {
     MlpNorm norm;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; renumber so that
-// consecutive tiles (neighbouring x segments, which share partial cache
-// lines on odd steps) land on the same XCD and its L2.
-__device__ __forceinline__ int xcd_tile(int bid, int G)
-{
-    const int k = G >> 3, r = G & 7;
-    const int x = bid & 7, y = bid >> 3;
-    return (x < r) ? x * (k + 1) + y : r * (k + 1) + (x - r) * k + y;
-}
-
 struct Cell {
     int x, y, z;
     uint32_t i;
@@ -153,17 +145,22 @@ __device__ __forceinline__ Cell cell_at(const Geo& g, int x, int y, int z)
     return c;
 }
 
+// Tiles cover the caller's cells only (halo cells are solid and never
+// collide).  Workgroups go to the XCDs in the hardware's round-robin order,
+// which keeps all of them on one narrow band of each plane.
 __device__ __forceinline__ Cell cell_of(const Geo& g)
 {
     Cell c;
-    const int t = xcd_tile((int)blockIdx.x, g.ntiles);
+    const int t = (int)blockIdx.x;
     const int tx = t % g.ntx;
     const int rem = t / g.ntx;
     const int ty = rem % g.nty;
-    c.z = rem / g.nty;
-    c.x = (tx << g.bx_log2) + ((int)threadIdx.x & ((1 << g.bx_log2) - 1));
-    c.y = ty * g.by + ((int)threadIdx.x >> g.bx_log2);
-    c.inside = c.x < g.nx && c.y < g.ny;
+    c.z = g.pz + rem / g.nty;
+    const int ux = (tx << g.bx_log2) + ((int)threadIdx.x & ((1 << g.bx_log2) - 1));
+    const int uy = ty * g.by + ((int)threadIdx.x >> g.bx_log2);
+    c.x = g.px + ux;
+    c.y = g.py + uy;
+    c.inside = ux < g.ux && uy < g.uy;
     c.i = (uint32_t)(c.x + g.nx * (c.y + g.ny * c.z));
     set_deltas(g, c);
     return c;
@@ -279,28 +276,24 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
 {
     using R = typename ST::R;
     const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
-    if (!FLAGS || cls == CL_FLUID) {
+    // node word of a boundary cell (0 for plain fluid): skip masks and walls
+    const uint64_t nw = (FLAGS && cls != CL_FLUID) ? a.node[c.i] : 0ull;
+    const uint32_t S = (uint32_t)(nw & MASK27);
+    const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
+    const bool wall = (nw >> 63) != 0;
+    // streaming writes, shared by every lane of the wave (boundary lanes drop
+    // the writes their mask skips); an inlet cell streams its pre-collision
+    // state (StreamingAB.h:62-65)
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (line_aligned(ODD, q)) store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+    for (int q = 0; q < Q; ++q)
+        if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
+            store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (!line_aligned(ODD, q)) store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
-    } else {
-        const uint64_t nw = a.node[c.i];
-        const uint32_t S = (uint32_t)(nw & MASK27);
-        const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
-        const bool wall = (nw >> 63) != 0;
+    for (int q = 0; q < Q; ++q)
+        if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
+            store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
+    if (FLAGS && cls != CL_FLUID) {
         if (inlet) {
-            // stream the pre-collision state (StreamingAB.h:62-65)
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
-                    store_out<ST, ODD, true>(a.p, a.rest, c, q, f[q], hsr);
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
-                    store_out<ST, ODD, false>(a.p, a.rest, c, q, f[q], hsr);
             if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                 R st[Q];
 #pragma unroll
@@ -310,14 +303,6 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
                 for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
             }
         } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
-                    store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
-                    store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 if ((S >> q) & 1u) {
@@ -428,28 +413,24 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         }
 
         const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
-        if (!FLAGS || cls == CL_FLUID) {
+        // node word of a boundary cell (0 for plain fluid): skip masks and walls
+        const uint64_t nw = (FLAGS && cls != CL_FLUID) ? a.node[c.i] : 0ull;
+        const uint32_t S = (uint32_t)(nw & MASK27);
+        const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
+        const bool wall = (nw >> 63) != 0;
+        // streaming writes, shared by every lane of the wave (boundary lanes drop
+        // the writes their mask skips); an inlet cell streams its pre-collision
+        // state (StreamingAB.h:62-65)
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (line_aligned(ODD, q)) store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+        for (int q = 0; q < Q; ++q)
+            if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
+                store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!line_aligned(ODD, q)) store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
-        } else {
-            const uint64_t nw = a.node[c.i];
-            const uint32_t S = (uint32_t)(nw & MASK27);
-            const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
-            const bool wall = (nw >> 63) != 0;
+        for (int q = 0; q < Q; ++q)
+            if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
+                store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
+        if (FLAGS && cls != CL_FLUID) {
             if (inlet) {
-                // stream the pre-collision state (StreamingAB.h:62-65)
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
-                        store_out<ST, ODD, true>(a.p, a.rest, c, q, f[q], hsr);
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
-                        store_out<ST, ODD, false>(a.p, a.rest, c, q, f[q], hsr);
                 if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
                     R st[Q];
 #pragma unroll
@@ -459,14 +440,6 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                     for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
                 }
             } else {
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u) && line_aligned(ODD, q))
-                        store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
-#pragma unroll
-                for (int q = 0; q < Q; ++q)
-                    if (!((Kk >> q) & 1u) && !line_aligned(ODD, q))
-                        store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     if ((S >> q) & 1u) {
