@@ -1,10 +1,10 @@
 """Size-independent properties at the full workload sizes (the oracle is too
-slow there): the decaying-turbulence lattice at 256^3 and the Taylor-Green
-lattice at 384^3.
+slow there): the decaying-turbulence lattice at 256^3, the Taylor-Green
+lattice at 384^3 and the obstacle workload's 512 x 256 x 256 shape.
 
 - Translation invariance: every cell runs the same arithmetic, so on an
   index-wrap periodic lattice a shifted initial field gives the shifted
-  result bit for bit.  This exercises the tile mapping, the XCD remap, the
+  result bit for bit.  This exercises the tile mapping, the halo layout, the
   wrap deltas and the 32-bit offsets over the whole lattice, on even and odd
   steps.
 - Conservation: with the true opposite table and exact weights (compat = 0)
@@ -56,6 +56,29 @@ def test_translation_invariance_384_taylor_green(gpu):
     a, b = _shifted_runs(n, abi.STORAGE_BF16DEV, (rho, ux, uy, uz), 5, compat=0, tau=3 * nu + 0.5)
     for k in range(4):
         assert np.array_equal(np.roll(a[k], SHIFT, axis=(0, 1, 2)), b[k]), k
+
+
+def test_translation_invariance_haloed_512x256x256(gpu):
+    """The obstacle workload's lattice shape, 512 x 256 x 256 with a solid halo
+    along x (whole-cache-line x halo, boundary cells at both x ends): shifts
+    along the wrapped y and z axes give the shifted result bit for bit."""
+    nx, ny, nz = 512, 256, 256
+    flags = np.ones((nz, ny, nx), np.int32)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij", sparse=True)
+    ux = 0.03 * np.sin(2 * np.pi * y / ny) * np.cos(2 * np.pi * z / nz) * np.sin(np.pi * (x + 0.5) / nx)
+    uy = 0.02 * np.cos(2 * np.pi * x / nx) * np.ones((nz, ny, nx))
+    out = []
+    for shift in ((0, 0), (5, 3)):
+        lat = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+                                c_les=0.17, storage=abi.STORAGE_F32, arith=abi.ARITH_FAST, compat=0,
+                                periodic=(0, 1, 1), tau=0.52))
+        lat.init_equilibrium(None, np.roll(ux, shift, axis=(0, 1)), np.roll(uy, shift, axis=(0, 1)), None)
+        lat.step(6)
+        out.append(lat.macroscopic())
+        lat.close()
+    for k in range(4):
+        assert np.array_equal(np.roll(out[0][k], (5, 3), axis=(0, 1)), out[1][k]), k
+    assert np.all(np.isfinite(out[0][1]))
 
 
 def test_conservation_fp64_256(gpu):
