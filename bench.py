@@ -72,9 +72,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_lattice(n, storage, fields, arith=abi.ARITH_FAST):
+def make_lattice(n, storage, fields, arith=abi.ARITH_FAST, collision=abi.COLLISION_CUM2015):
     flags = np.ones((n, n, n), np.int32)
-    d = make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY, storage=storage,
+    d = make_desc(flags, [], collision=collision, closure=abi.CLOSURE_SMAGORINSKY, storage=storage,
                   arith=arith, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=TAU0, c_les=C_LES)
     L = Lattice(d)
     L.init_equilibrium(None, *fields)
@@ -93,7 +93,8 @@ def closure_network(seed=7):
     return p, norm
 
 
-def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False, refresh=1):
+def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False, refresh=1,
+            collision=abi.COLLISION_CUM2015):
     if mlp:
         flags = np.ones((n, n, n), np.int32)
         L = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_MLP, storage=storage,
@@ -103,7 +104,7 @@ def run_gpu(n, storage, fields, steps, warmup, arith=abi.ARITH_FAST, mlp=False, 
         L.set_closure_refresh(refresh)
         L.init_equilibrium(None, *fields)
     else:
-        L = make_lattice(n, storage, fields, arith)
+        L = make_lattice(n, storage, fields, arith, collision)
     L.step(warmup)
     L.sync()
     t0 = time.perf_counter()
@@ -207,6 +208,16 @@ def main():
                               "steps": vsteps, "bytes_per_cell": 54 * sb,
                               "vs_f32_mlups": round(r["mlups"] / res["mlups"], 3)}
             log(f"{name}: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
+        # the Geier-2017 operator (limited third-order rates, fourth-order
+        # A/B terms) on the same workload
+        for name in ("f32", "f16dev_sr"):
+            r = run_gpu(n, STORAGES[name], fields, vsteps, 20, collision=abi.COLLISION_CUM2017)
+            sb = abi.STORAGE_BYTES[STORAGES[name]]
+            ach = cells * 54 * sb / (r["kernel_ms"] * 1e-3) / 1e9
+            variants[f"{name}_cum2017"] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                                           "achieved_gbs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                                           "steps": vsteps, "vs_f32_mlups": round(r["mlups"] / res["mlups"], 3)}
+            log(f"{name} cum2017: {r['mlups']:.0f} MLUPS, {ach:.0f} GB/s")
         # learned closure fused into the same step (FP32 storage), network
         # evaluated every step and every 10 steps (coefficient held between)
         for every in (1, 10):

@@ -42,8 +42,14 @@ typedef enum lbm_status {
     LBM_E_UNSUPPORTED = 6   /* valid request that this build does not implement */
 } lbm_status;
 
-/* Collision operator (reference: collisions/CollisionSRT*.h, CollisionCumD3Q27*2015.h). */
-enum { LBM_COLLISION_SRT = 0, LBM_COLLISION_CUM2015 = 1 };
+/* Collision operator (reference: collisions/CollisionSRT*.h,
+ * CollisionCumD3Q27Turbulent2015.h, CollisionCumD3Q27Turbulent2017.h,
+ * CollisionCumD3Q27TurbulentCombined.h).  CUM2017 is the parametrized cumulant
+ * (omega_3..5 from omega_1 with magnitude limiters, fourth-order A/B terms);
+ * COMBINED collides cells whose flag is 1 (fluid, no solid neighbour) with the
+ * 2015 operator and every other non-solid cell with the 2017 one
+ * (Combined.h:1066-1078).  The learned closure runs with SRT and CUM2015. */
+enum { LBM_COLLISION_SRT = 0, LBM_COLLISION_CUM2015 = 1, LBM_COLLISION_CUM2017 = 2, LBM_COLLISION_COMBINED = 3 };
 
 /* Relaxation-rate closure (reference: turbulence/OmegaNo.h, turbulence/OmegaLES.h).
  * LBM_CLOSURE_MLP is the learned sub-grid closure (no reference counterpart). */
@@ -72,8 +78,13 @@ enum { LBM_ARITH_LITERAL = 0, LBM_ARITH_FAST = 1 };
 
 /* Compatibility switches reproducing reference quirks. */
 #define LBM_COMPAT_CREV20    0x1u  /* D3Q27.h:120-124 c_rev[20] = 21 (correct: 19)          */
-#define LBM_COMPAT_F32CONST  0x2u  /* D3Q27.h:113-118 float weights; Cum2015.h:280-282 1.f/3.f */
-#define LBM_COMPAT_REFERENCE (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST)
+#define LBM_COMPAT_F32CONST  0x2u  /* D3Q27.h:113-118 float weights; Cum2015.h:280-282 1.f/3.f;
+                                      Cum2017.h:244-246, 349-362 0.01f, 2.f/3.f, 4.f/3.f    */
+#define LBM_COMPAT_LIM210    0x4u  /* Cum2017.h:254-255, 264-265 the 210/012 limiters read
+                                      |C_120 +- C_102| (correct: |C_210 +- C_012|)          */
+#define LBM_COMPAT_TRACE17   0x8u  /* Cum2017.h:295 subtracts (rho - 1), Combined.h:798 (rho + 1)
+                                      from the trace in D_x u (correct: k_000, as Cum2015.h:253) */
+#define LBM_COMPAT_REFERENCE (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST | LBM_COMPAT_LIM210 | LBM_COMPAT_TRACE17)
 
 /* Boundary-node kinds.  The flag grid keeps the reference encoding
  * (geometryMesherBoundary.h): 0 solid, >0 fluid (1 + number of solid

@@ -19,13 +19,13 @@ LBM_OK, LBM_E_ARG, LBM_E_HIP, LBM_E_OOM, LBM_E_NAN, LBM_E_STATE, LBM_E_UNSUPPORT
 STATUS_NAMES = {0: "LBM_OK", 1: "LBM_E_ARG", 2: "LBM_E_HIP", 3: "LBM_E_OOM", 4: "LBM_E_NAN",
                 5: "LBM_E_STATE", 6: "LBM_E_UNSUPPORTED"}
 
-COLLISION_SRT, COLLISION_CUM2015 = 0, 1
+COLLISION_SRT, COLLISION_CUM2015, COLLISION_CUM2017, COLLISION_COMBINED = 0, 1, 2, 3
 CLOSURE_NONE, CLOSURE_SMAGORINSKY, CLOSURE_MLP = 0, 1, 2
 STORAGE_F64, STORAGE_F32, STORAGE_F16DEV, STORAGE_BF16DEV = 0, 1, 2, 3
 STORAGE_F16DEV_SR, STORAGE_BF16DEV_SR = 4, 5   # stochastic rounding in the step's stores
 ARITH_LITERAL, ARITH_FAST = 0, 1
-COMPAT_CREV20, COMPAT_F32CONST = 0x1, 0x2
-COMPAT_REFERENCE = COMPAT_CREV20 | COMPAT_F32CONST
+COMPAT_CREV20, COMPAT_F32CONST, COMPAT_LIM210, COMPAT_TRACE17 = 0x1, 0x2, 0x4, 0x8
+COMPAT_REFERENCE = COMPAT_CREV20 | COMPAT_F32CONST | COMPAT_LIM210 | COMPAT_TRACE17
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
 
 STORAGE_BYTES = {STORAGE_F64: 8, STORAGE_F32: 4, STORAGE_F16DEV: 2, STORAGE_BF16DEV: 2, STORAGE_F16DEV_SR: 2,

@@ -88,7 +88,7 @@ def build_tests():
         return None
     out = os.path.join(nat, "libcollide_host.so")
     _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-          f"-I{CSRC}", "-o", out, src])
+          f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}", "-o", out, src])
     return out
 
 

@@ -17,6 +17,7 @@
 
 #include <cmath>
 
+#include "../../include/lbm_core.h"
 #include "d3q27.hpp"
 
 namespace lbm {
@@ -34,7 +35,38 @@ template <class R>
 struct Consts {
     R w[4];     // weight per class: rest, axis, edge, corner
     R c13;      // the 1/3 factor of Eq. G2015 61-73 (reference: 1.f/3.f)
+    R c23, c43; // 2/3 and 4/3 of Eq. G2017 43-45 (reference: 2.f/3.f, 4.f/3.f)
+    R lam;      // limiter parameter lambda_3..5 of Geier 2017 (reference: 0.01f)
+    int lim210;   // the 210/012 limiters read |C_120 +- C_102| (reference quirk)
+    int trace17;  // 2017 bulk term subtracts rho -+ 1 instead of k_000 (reference quirk)
 };
+
+// Constants for a set of LBM_COMPAT_* bits.  Float literals of the reference
+// (D3Q27.h:113-118, CollisionCumD3Q27Turbulent2015.h:280-306,
+// CollisionCumD3Q27Turbulent2017.h:244-246, 349-362) keep their float
+// rounding under LBM_COMPAT_F32CONST.
+template <class R>
+inline Consts<R> make_consts(uint32_t compat)
+{
+    Consts<R> K;
+    const bool f32 = (compat & LBM_COMPAT_F32CONST) != 0;
+    K.w[0] = f32 ? (R)(8.f / 27.f) : (R)(8.0 / 27.0);
+    K.w[1] = f32 ? (R)(2.f / 27.f) : (R)(2.0 / 27.0);
+    K.w[2] = f32 ? (R)(1.f / 54.f) : (R)(1.0 / 54.0);
+    K.w[3] = f32 ? (R)(1.f / 216.f) : (R)(1.0 / 216.0);
+    K.c13 = f32 ? (R)(1.f / 3.f) : (R)(1.0 / 3.0);
+    K.c23 = f32 ? (R)(2.f / 3.f) : (R)(2.0 / 3.0);
+    K.c43 = f32 ? (R)(4.f / 3.f) : (R)(4.0 / 3.0);
+    K.lam = f32 ? (R)0.01f : (R)0.01;
+    K.lim210 = (compat & LBM_COMPAT_LIM210) != 0;
+    K.trace17 = (compat & LBM_COMPAT_TRACE17) != 0;
+    return K;
+}
+
+// Variants of the cumulant operator: the Geier-2015 operator, the 2017
+// operator of CollisionCumD3Q27Turbulent2017.h and the 2017 branch of
+// CollisionCumD3Q27TurbulentCombined.h (they differ only in the bulk-term quirk).
+enum { CV_2015 = 0, CV_2017 = 1, CV_2017C = 2 };
 
 template <class R>
 LBM_HD R sqrt_r(R x)
@@ -228,11 +260,13 @@ LBM_HD void corr45(const R* k, R rho, R* out /* 211 121 112 220 022 202 122 212 
                    M_(1, 2, 0), M_(0, 1, 1), M_(2, 1, 0)) / rho;
 }
 
-// Geier-2015 cumulant collision in the reference's operation order.
+// Cumulant collision in the reference's operation order: Geier 2015
+// (CollisionCumD3Q27Turbulent2015.h:84-485) or Geier 2017
+// (CollisionCumD3Q27Turbulent2017.h:84-539; Combined.h:84-542 for CV_2017C).
 // f: in/out populations; rho, u: the macroscopic fields the reference passes
 // in (moments of the previous step); omg: omega_1.
-template <class R>
-LBM_HD void cum2015_literal(const Consts<R>& K, R* f, R rho, R ux, R uy, R uz, R omg)
+template <int V, class R>
+LBM_HD void cum_literal(const Consts<R>& K, R* f, R rho, R ux, R uy, R uz, R omg)
 {
     LBM_FP_STRICT
     R k[27];
@@ -253,45 +287,128 @@ LBM_HD void cum2015_literal(const Consts<R>& K, R* f, R rho, R ux, R uy, R uz, R
     const R C122 = M_(1, 2, 2) - c45[6], C212 = M_(2, 1, 2) - c45[7], C221 = M_(2, 2, 1) - c45[8];
     const R C222 = M_(2, 2, 2) - six_a(k) / rho + six_b(k) / rho / rho;
 
-    // relaxation (omega_2..omega_10 = 1 as configured by the reference)
-    const R w1 = omg, w2 = R(1), w3 = R(1), w4 = R(1), w5 = R(1), w6 = R(1), w7 = R(1), w8 = R(1), w9 = R(1),
-            w10 = R(1);
-    const R Dxu = -w1 * R(0.5) / rho * (R(2) * C200 - C020 - C002) - w2 * R(0.5) / rho * (C200 + C020 + C002 - k000);
-    const R Dyv = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C020);
-    const R Dzw = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C002);
-    const R e61 = (R(1) - w1) * (C200 - C020) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uy * uy * Dyv);
-    const R e64 = (R(1) - w1) * (C200 - C002) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uz * uz * Dzw);
-    const R e65 = k000 * w2 + (R(1) - w2) * (C200 + C020 + C002) -
-                  R(3) * rho * (R(1) - w2 / R(2)) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
-    const R e71 = (R(1) - w6) * (C220 - R(2) * C202 + C022);
-    const R e72 = (R(1) - w6) * (C220 + C202 - R(2) * C022);
-    const R e73 = (R(1) - w7) * (C220 + C202 + C022);
-
     R s[27];
 #define S_(i, j, l) s[9 * (i) + 3 * (j) + (l)]
+    R cs220, cs202, cs022, cs211, cs121, cs112, cs221, cs212, cs122, cs222;
     S_(0, 0, 0) = k000;
     S_(1, 0, 0) = -M_(1, 0, 0);
     S_(0, 1, 0) = -M_(0, 1, 0);
     S_(0, 0, 1) = -M_(0, 0, 1);
-    S_(1, 1, 0) = (R(1) - w1) * C110;
-    S_(1, 0, 1) = (R(1) - w1) * C101;
-    S_(0, 1, 1) = (R(1) - w1) * C011;
-    S_(2, 0, 0) = K.c13 * (e61 + e64 + e65);
-    S_(0, 2, 0) = K.c13 * (e64 - R(2) * e61 + e65);
-    S_(0, 0, 2) = K.c13 * (e61 - R(2) * e64 + e65);
-    S_(1, 2, 0) = (-C102 - C120) * w3 * R(0.5) + (C102 - C120) * w4 * R(0.5) + C120;
-    S_(1, 0, 2) = (-C102 - C120) * w3 * R(0.5) + (-C102 + C120) * w4 * R(0.5) + C102;
-    S_(2, 1, 0) = (-C012 - C210) * w3 * R(0.5) + (C012 - C210) * w4 * R(0.5) + C210;
-    S_(0, 1, 2) = (-C012 - C210) * w3 * R(0.5) + (-C012 + C210) * w4 * R(0.5) + C012;
-    S_(0, 2, 1) = (-C021 - C201) * w3 * R(0.5) + (-C021 + C201) * w4 * R(0.5) + C021;
-    S_(2, 0, 1) = (-C021 - C201) * w3 * R(0.5) + (C021 - C201) * w4 * R(0.5) + C201;
-    S_(1, 1, 1) = (R(1) - w5) * C111;
-    const R cs220 = K.c13 * (e71 + e72 + e73);
-    const R cs202 = K.c13 * (-e71 + e73);
-    const R cs022 = K.c13 * (-e72 + e73);
-    const R cs211 = (R(1) - w8) * C211, cs121 = (R(1) - w8) * C121, cs112 = (R(1) - w8) * C112;
-    const R cs221 = (R(1) - w9) * C221, cs212 = (R(1) - w9) * C212, cs122 = (R(1) - w9) * C122;
-    const R cs222 = (R(1) - w10) * C222;
+    if constexpr (V == CV_2015) {
+        // relaxation (omega_2..omega_10 = 1 as configured by the reference)
+        const R w1 = omg, w2 = R(1), w3 = R(1), w4 = R(1), w5 = R(1), w6 = R(1), w7 = R(1), w8 = R(1), w9 = R(1),
+                w10 = R(1);
+        const R Dxu =
+            -w1 * R(0.5) / rho * (R(2) * C200 - C020 - C002) - w2 * R(0.5) / rho * (C200 + C020 + C002 - k000);
+        const R Dyv = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C020);
+        const R Dzw = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C002);
+        const R e61 =
+            (R(1) - w1) * (C200 - C020) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uy * uy * Dyv);
+        const R e64 =
+            (R(1) - w1) * (C200 - C002) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uz * uz * Dzw);
+        const R e65 = k000 * w2 + (R(1) - w2) * (C200 + C020 + C002) -
+                      R(3) * rho * (R(1) - w2 / R(2)) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
+        const R e71 = (R(1) - w6) * (C220 - R(2) * C202 + C022);
+        const R e72 = (R(1) - w6) * (C220 + C202 - R(2) * C022);
+        const R e73 = (R(1) - w7) * (C220 + C202 + C022);
+        S_(1, 1, 0) = (R(1) - w1) * C110;
+        S_(1, 0, 1) = (R(1) - w1) * C101;
+        S_(0, 1, 1) = (R(1) - w1) * C011;
+        S_(2, 0, 0) = K.c13 * (e61 + e64 + e65);
+        S_(0, 2, 0) = K.c13 * (e64 - R(2) * e61 + e65);
+        S_(0, 0, 2) = K.c13 * (e61 - R(2) * e64 + e65);
+        S_(1, 2, 0) = (-C102 - C120) * w3 * R(0.5) + (C102 - C120) * w4 * R(0.5) + C120;
+        S_(1, 0, 2) = (-C102 - C120) * w3 * R(0.5) + (-C102 + C120) * w4 * R(0.5) + C102;
+        S_(2, 1, 0) = (-C012 - C210) * w3 * R(0.5) + (C012 - C210) * w4 * R(0.5) + C210;
+        S_(0, 1, 2) = (-C012 - C210) * w3 * R(0.5) + (-C012 + C210) * w4 * R(0.5) + C012;
+        S_(0, 2, 1) = (-C021 - C201) * w3 * R(0.5) + (-C021 + C201) * w4 * R(0.5) + C021;
+        S_(2, 0, 1) = (-C021 - C201) * w3 * R(0.5) + (C021 - C201) * w4 * R(0.5) + C201;
+        S_(1, 1, 1) = (R(1) - w5) * C111;
+        cs220 = K.c13 * (e71 + e72 + e73);
+        cs202 = K.c13 * (-e71 + e73);
+        cs022 = K.c13 * (-e72 + e73);
+        cs211 = (R(1) - w8) * C211, cs121 = (R(1) - w8) * C121, cs112 = (R(1) - w8) * C112;
+        cs221 = (R(1) - w9) * C221, cs212 = (R(1) - w9) * C212, cs122 = (R(1) - w9) * C122;
+        cs222 = (R(1) - w10) * C222;
+    } else {
+        // relaxation rates, Geier 2017 (Turbulent2017.h:240-291): omega_2 = 1,
+        // omega_3..5 from omega_1 with magnitude limiters, omega_6..10 = 1
+        const R w1 = omg, w2 = R(1);
+        const R lam = K.lam;
+        const R w3 = R(8) * (w1 - R(2)) * (w2 * (R(3) * w1 - R(1)) - R(5) * w1) /
+                     (R(8) * (R(5) - R(2) * w1) * w1 + w2 * (R(8) + w1 * (R(9) * w1 - R(26))));
+        const R a210p = K.lim210 ? C120 + C102 : C210 + C012;
+        const R a210m = K.lim210 ? C120 - C102 : C210 - C012;
+        const R w120p102 = w3 + (R(1) - w3) * std::fabs(C120 + C102) / (rho * lam + std::fabs(C120 + C102));
+        const R w210p012 = w3 + (R(1) - w3) * std::fabs(a210p) / (rho * lam + std::fabs(a210p));
+        const R w201p021 = w3 + (R(1) - w3) * std::fabs(C201 + C021) / (rho * lam + std::fabs(C201 + C021));
+        const R w4 = R(8) * (w1 - R(2)) * (w1 + w2 * (R(3) * w1 - R(7))) /
+                     (w2 * (R(56) - R(42) * w1 + R(8) * w1 * w1) - R(8) * w1);
+        const R w120m102 = w4 + (R(1) - w4) * std::fabs(C120 - C102) / (rho * lam + std::fabs(C120 - C102));
+        const R w210m012 = w4 + (R(1) - w4) * std::fabs(a210m) / (rho * lam + std::fabs(a210m));
+        const R w201m021 = w4 + (R(1) - w4) * std::fabs(C201 - C021) / (rho * lam + std::fabs(C201 - C021));
+        const R w5 = R(24) * (w1 - R(2)) *
+                     (R(4) * w1 * w1 + w1 * w2 * (R(18) - R(13) * w1) + w2 * w2 * (R(2) + w1 * (R(6) * w1 - R(11)))) /
+                     (R(16) * w1 * w1 * (w1 - R(6)) - R(2) * w1 * w2 * (R(216) + R(5) * w1 * (R(9) * w1 - R(46))) +
+                      w2 * w2 * (w1 * (R(3) * w1 - R(10)) * (R(15) * w1 - R(28)) - R(48)));
+        const R w111 = w5 + (R(1) - w5) * std::fabs(C111) / (rho * lam + std::fabs(C111));
+        const R w6 = R(1), w7 = R(1), w8 = R(1), w9 = R(1), w10 = R(1);
+        const R A = (R(4) * w1 * w1 + R(2) * w1 * w2 * (w1 - R(6)) + w2 * w2 * (w1 * (R(10) - R(3) * w1) - R(4))) /
+                    (w1 - w2) / (w2 * (R(2) + R(3) * w1) - R(8) * w1);
+        const R B = (R(4) * w1 * w2 * (R(9) * w1 - R(16)) - R(4) * w1 * w1 -
+                     R(2) * w2 * w2 * (R(2) + R(9) * w1 * (w1 - R(2)))) /
+                    R(3) / (w1 - w2) / (w2 * (R(2) + R(3) * w1) - R(8) * w1);
+        // bulk term: k_000 (corrected) or the reference's rho - 1 / rho + 1
+        R tr = k000;
+        if (K.trace17) tr = V == CV_2017 ? R(-1) + rho : -(R(-1) - rho);
+        const R Dxu = -w1 * R(0.5) / rho * (R(2) * C200 - C020 - C002) - w2 * R(0.5) / rho * (C200 + C020 + C002 - tr);
+        const R Dyv = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C020);
+        const R Dzw = Dxu + R(3) * w1 * R(0.5) / rho * (C200 - C002);
+        const R DxvDyu = -R(3) * w1 / rho * C110;
+        const R DxwDzu = -R(3) * w1 / rho * C101;
+        const R DywDzv = -R(3) * w1 / rho * C011;
+        const R e61 =
+            (R(1) - w1) * (C200 - C020) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uy * uy * Dyv);
+        const R e64 =
+            (R(1) - w1) * (C200 - C002) - R(3) * rho * (R(1) - w1 * R(0.5)) * (ux * ux * Dxu - uz * uz * Dzw);
+        const R e65 = k000 * w2 + (R(1) - w2) * (C200 + C020 + C002) -
+                      R(3) * rho * (R(1) - w2 / R(2)) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
+        S_(1, 1, 0) = (R(1) - w1) * C110;
+        S_(1, 0, 1) = (R(1) - w1) * C101;
+        S_(0, 1, 1) = (R(1) - w1) * C011;
+        S_(2, 0, 0) = K.c13 * (e61 + e64 + e65);
+        S_(0, 2, 0) = K.c13 * (e64 - R(2) * e61 + e65);
+        S_(0, 0, 2) = K.c13 * (e61 - R(2) * e64 + e65);
+        // third order, Eq. G2017 117-122 and 42 (:331-346)
+        const R e117 = (R(1) - w120p102) * (C120 + C102);
+        const R e118 = (R(1) - w210p012) * (C210 + C012);
+        const R e119 = (R(1) - w201p021) * (C201 + C021);
+        const R e120 = (R(1) - w120m102) * (C120 - C102);
+        const R e121 = (R(1) - w210m012) * (C210 - C012);
+        const R e122 = (R(1) - w201m021) * (C201 - C021);
+        S_(1, 2, 0) = R(0.5) * (e120 + e117);
+        S_(1, 0, 2) = R(0.5) * (-e120 + e117);
+        S_(2, 1, 0) = R(0.5) * (e121 + e118);
+        S_(0, 1, 2) = R(0.5) * (-e121 + e118);
+        S_(0, 2, 1) = R(0.5) * (-e122 + e119);
+        S_(2, 0, 1) = R(0.5) * (e122 + e119);
+        S_(1, 1, 1) = (R(1) - w111) * C111;
+        // fourth order, Eq. G2017 43-48 (:349-365)
+        const R e43 = K.c23 * (R(1) / w1 - R(0.5)) * w6 * A * rho * (Dxu - R(2) * Dyv + Dzw) +
+                      (R(1) - w6) * (C220 - R(2) * C202 + C022);
+        const R e44 = K.c23 * (R(1) / w1 - R(0.5)) * w6 * A * rho * (Dxu + Dyv - R(2) * Dzw) +
+                      (R(1) - w6) * (C220 + C202 - R(2) * C022);
+        const R e45 = -K.c43 * (R(1) / w1 - R(0.5)) * w7 * A * rho * (Dxu + Dyv + Dzw) +
+                      (R(1) - w7) * (C220 + C202 + C022);
+        cs220 = K.c13 * (e43 + e44 + e45);
+        cs202 = K.c13 * (-e43 + e45);
+        cs022 = K.c13 * (-e44 + e45);
+        cs211 = -K.c13 * (R(1) / w1 - R(0.5)) * w8 * B * rho * DywDzv + (R(1) - w8) * C211;
+        cs121 = -K.c13 * (R(1) / w1 - R(0.5)) * w8 * B * rho * DxwDzu + (R(1) - w8) * C121;
+        cs112 = -K.c13 * (R(1) / w1 - R(0.5)) * w8 * B * rho * DxvDyu + (R(1) - w8) * C112;
+        cs221 = (R(1) - w9) * C221, cs212 = (R(1) - w9) * C212, cs122 = (R(1) - w9) * C122;
+        cs222 = (R(1) - w10) * C222;
+    }
 
     // back to central moments, Eq. G2015 81-84
     R b45[9];
@@ -504,6 +621,131 @@ LBM_HD void cum2015_fast(const Consts<R>& K, const Cen2<R>& c, R m0, R rho, R ux
         inv_gen(Y[3 * ab], Y[3 * ab + 1], Y[3 * ab + 2], uz, uzz, uzm, uzp, F[3 * ab], F[3 * ab + 1], F[3 * ab + 2]);
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = F[CELL3(q)];
+}
+
+// Reduced Geier-2017 collision (fast variant).  omega_6..omega_10 = 1 as in
+// the reference (Turbulent2017.h:279-283), so the post-collision cumulants of
+// order 5 and 6 vanish and those of order 4 are the A/B gradient terms alone;
+// order 3 relaxes with the limited omega_3..5 and so needs the pre-collision
+// third-order central moments, which come from the forward transform of f
+// (the compiler keeps only the passes they need).  Orders 0..2 come from c,
+// the same central moments the Smagorinsky rate was formed from.
+template <int V, class R>
+LBM_HD void cum2017_fast(const Consts<R>& K, const R* fin, const Cen2<R>& c, R m0, R rho, R ux, R uy, R uz, R omg,
+                         R* f)
+{
+    R k[27];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) k[CELL3(q)] = fin[q];
+    fwd_all(k, ux, uy, uz);
+#define M_(i, j, l) k[9 * (i) + 3 * (j) + (l)]
+    const R C120 = M_(1, 2, 0), C102 = M_(1, 0, 2), C210 = M_(2, 1, 0), C012 = M_(0, 1, 2);
+    const R C201 = M_(2, 0, 1), C021 = M_(0, 2, 1), C111 = M_(1, 1, 1);
+#undef M_
+    const R ir = R(1) / rho;
+    const R C200 = c.kxx, C020 = c.kyy, C002 = c.kzz;
+
+    // rates and limiters (Turbulent2017.h:240-291) with omega_2 = 1 substituted
+    const R w1 = omg;
+    const R w3 = -R(8) * (w1 - R(2)) * (R(2) * w1 + R(1)) / (R(8) + w1 * (R(14) - R(7) * w1));
+    const R w4 = R(8) * (w1 - R(2)) * (R(4) * w1 - R(7)) / (R(56) - R(50) * w1 + R(8) * w1 * w1);
+    const R w5 = R(24) * (w1 - R(2)) * (R(2) + w1 * (R(7) - R(3) * w1)) /
+                 (R(16) * w1 * w1 * (w1 - R(6)) - R(2) * w1 * (R(216) + R(5) * w1 * (R(9) * w1 - R(46))) +
+                  (w1 * (R(3) * w1 - R(10)) * (R(15) * w1 - R(28)) - R(48)));
+    const R den = (w1 - R(1)) * (R(2) - R(5) * w1);
+    const R A = (R(4) * w1 * w1 + R(2) * w1 * (w1 - R(6)) + w1 * (R(10) - R(3) * w1) - R(4)) / den;
+    const R B = (R(4) * w1 * (R(9) * w1 - R(16)) - R(4) * w1 * w1 - R(2) * (R(2) + R(9) * w1 * (w1 - R(2)))) /
+                (R(3) * den);
+    const R rl = rho * K.lam;
+    auto lim = [rl](R w, R a) {
+        const R m = std::fabs(a);
+        return (R(1) - w) * rl / (rl + m);   // 1 - (w + (1 - w) |a| / (rl + |a|))
+    };
+    const R p120 = C120 + C102, m120 = C120 - C102;
+    const R p210 = C210 + C012, m210 = C210 - C012;
+    const R p201 = C201 + C021, m201 = C201 - C021;
+    const R e117 = lim(w3, p120) * p120;
+    const R e118 = lim(w3, K.lim210 ? p120 : p210) * p210;
+    const R e119 = lim(w3, p201) * p201;
+    const R e120 = lim(w4, m120) * m120;
+    const R e121 = lim(w4, K.lim210 ? m120 : m210) * m210;
+    const R e122 = lim(w4, m201) * m201;
+    const R s120 = R(0.5) * (e120 + e117), s102 = R(0.5) * (e117 - e120);
+    const R s210 = R(0.5) * (e121 + e118), s012 = R(0.5) * (e118 - e121);
+    const R s201 = R(0.5) * (e122 + e119), s021 = R(0.5) * (e119 - e122);
+    const R s111 = lim(w5, C111) * C111;
+
+    // second order as in the 2015 operator, with the 2017 bulk term
+    R tr = m0;
+    if (K.trace17) tr = V == CV_2017 ? rho - R(1) : rho + R(1);
+    const R hw = R(0.5) * omg * ir;
+    const R Dxu = -hw * (R(2) * C200 - C020 - C002) - R(0.5) * ir * (C200 + C020 + C002 - tr);
+    const R Dyv = Dxu + R(3) * hw * (C200 - C020);
+    const R Dzw = Dxu + R(3) * hw * (C200 - C002);
+    const R om1 = R(1) - omg;
+    const R A3 = R(3) * rho * (R(1) - R(0.5) * omg);
+    const R xx = ux * ux * Dxu, yy = uy * uy * Dyv, zz = uz * uz * Dzw;
+    const R e61 = om1 * (C200 - C020) - A3 * (xx - yy);
+    const R e64 = om1 * (C200 - C002) - A3 * (xx - zz);
+    const R e65 = m0 - R(1.5) * rho * (xx + yy + zz);
+    const R s200 = K.c13 * (e61 + e64 + e65);
+    const R s020 = K.c13 * (e64 - R(2) * e61 + e65);
+    const R s002 = K.c13 * (e61 - R(2) * e64 + e65);
+    const R s110 = om1 * c.kxy, s101 = om1 * c.kxz, s011 = om1 * c.kyz;
+
+    // fourth-order cumulants, Eq. G2017 43-48 with omega_6..8 = 1: (1/omega_1 - 1/2) rho A, B
+    const R g = (R(1) / w1 - R(0.5)) * rho;
+    const R ga = g * A, gb = -K.c13 * g * B * (-R(3) * w1 * ir);
+    const R e43 = K.c23 * ga * (Dxu - R(2) * Dyv + Dzw);
+    const R e44 = K.c23 * ga * (Dxu + Dyv - R(2) * Dzw);
+    const R e45 = -K.c43 * ga * (Dxu + Dyv + Dzw);
+    const R c220 = K.c13 * (e43 + e44 + e45), c202 = K.c13 * (e45 - e43), c022 = K.c13 * (e45 - e44);
+    const R c211 = gb * c.kyz, c121 = gb * c.kxz, c112 = gb * c.kxy;
+
+    // back to central moments, Eq. G2015 81-84 (orders 5 and 6 from products)
+    R s[27];
+#define S_(i, j, l) s[9 * (i) + 3 * (j) + (l)]
+    S_(0, 0, 0) = m0;
+    S_(1, 0, 0) = -c.k100;
+    S_(0, 1, 0) = -c.k010;
+    S_(0, 0, 1) = -c.k001;
+    S_(2, 0, 0) = s200, S_(0, 2, 0) = s020, S_(0, 0, 2) = s002;
+    S_(1, 1, 0) = s110, S_(1, 0, 1) = s101, S_(0, 1, 1) = s011;
+    S_(1, 2, 0) = s120, S_(1, 0, 2) = s102, S_(2, 1, 0) = s210;
+    S_(0, 1, 2) = s012, S_(2, 0, 1) = s201, S_(0, 2, 1) = s021, S_(1, 1, 1) = s111;
+    S_(2, 1, 1) = c211 + (s200 * s011 + R(2) * s101 * s110) * ir;
+    S_(1, 2, 1) = c121 + (s020 * s101 + R(2) * s110 * s011) * ir;
+    S_(1, 1, 2) = c112 + (s002 * s110 + R(2) * s011 * s101) * ir;
+    S_(2, 2, 0) = c220 + (s020 * s200 + R(2) * s110 * s110) * ir;
+    S_(0, 2, 2) = c022 + (s002 * s020 + R(2) * s011 * s011) * ir;
+    S_(2, 0, 2) = c202 + (s200 * s002 + R(2) * s101 * s101) * ir;
+    S_(1, 2, 2) = (s020 * s102 + s002 * s120 + R(4) * s011 * s111 + R(2) * (s110 * s012 + s101 * s021)) * ir;
+    S_(2, 1, 2) = (s002 * s210 + s200 * s012 + R(4) * s101 * s111 + R(2) * (s011 * s201 + s110 * s102)) * ir;
+    S_(2, 2, 1) = (s200 * s021 + s020 * s201 + R(4) * s110 * s111 + R(2) * (s101 * s120 + s011 * s210)) * ir;
+    S_(2, 2, 2) = (R(4) * s111 * s111 + s200 * S_(0, 2, 2) + s020 * S_(2, 0, 2) + s002 * S_(2, 2, 0) +
+                   R(4) * (s011 * S_(2, 1, 1) + s101 * S_(1, 2, 1) + s110 * S_(1, 1, 2)) +
+                   R(2) * (s120 * s102 + s210 * s012 + s201 * s021)) * ir -
+                  (R(16) * s110 * s101 * s011 + R(4) * (s101 * s101 * s020 + s011 * s011 * s200 + s110 * s110 * s002) +
+                   R(2) * s200 * s020 * s002) * ir * ir;
+#undef S_
+    // x, y, z passes of the backward transform
+    const R uxx = ux * ux, uxm = uxx - ux, uxp = uxx + ux;
+    R X[27];
+#pragma unroll
+    for (int bc = 0; bc < 9; ++bc) inv_gen(s[bc], s[9 + bc], s[18 + bc], ux, uxx, uxm, uxp, X[bc], X[9 + bc], X[18 + bc]);
+    const R uyy = uy * uy, uym = uyy - uy, uyp = uyy + uy;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int oz = 0; oz < 3; ++oz)
+            inv_gen(X[9 * a + oz], X[9 * a + 3 + oz], X[9 * a + 6 + oz], uy, uyy, uym, uyp, s[9 * a + oz],
+                    s[9 * a + 3 + oz], s[9 * a + 6 + oz]);
+    const R uzz = uz * uz, uzm = uzz - uz, uzp = uzz + uz;
+#pragma unroll
+    for (int ab = 0; ab < 9; ++ab)
+        inv_gen(s[3 * ab], s[3 * ab + 1], s[3 * ab + 2], uz, uzz, uzm, uzp, X[3 * ab], X[3 * ab + 1], X[3 * ab + 2]);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = X[CELL3(q)];
 }
 
 // BGK with the closed-form equilibrium (fast variant).

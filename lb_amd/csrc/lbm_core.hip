@@ -89,26 +89,6 @@ static lbm_status dev_alloc(lbm_ctx* c, void** p, size_t bytes)
     return LBM_OK;
 }
 
-template <class R>
-static Consts<R> make_consts(uint32_t compat)
-{
-    Consts<R> K;
-    if (compat & LBM_COMPAT_F32CONST) {
-        K.w[0] = (R)(8.f / 27.f);
-        K.w[1] = (R)(2.f / 27.f);
-        K.w[2] = (R)(1.f / 54.f);
-        K.w[3] = (R)(1.f / 216.f);
-        K.c13 = (R)(1.f / 3.f);
-    } else {
-        K.w[0] = (R)(8.0 / 27.0);
-        K.w[1] = (R)(2.0 / 27.0);
-        K.w[2] = (R)(1.0 / 54.0);
-        K.w[3] = (R)(1.0 / 216.0);
-        K.c13 = (R)(1.0 / 3.0);
-    }
-    return K;
-}
-
 template <class ST>
 static void fill_common(const lbm_ctx* c, typename ST::T** p, typename ST::R* rest, Consts<typename ST::R>& K)
 {
@@ -253,7 +233,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         a.diag = (s == 0 && c->diag_next) ? 1 : 0;
         if constexpr (ST::sr) a.seed = step_seed((uint64_t)c->steps);
         const bool odd = (c->steps & 1) != 0;
-        if constexpr (sizeof(R) == 4 && !LIT) {
+        if constexpr (sizeof(R) == 4 && !LIT && COLL <= LBM_COLLISION_CUM2015) {
             if (mlp && c->cl_count % c->cl_every == 0) {
                 // network evaluated in this step; coefficients kept for the next ones
                 ma.s = a;
@@ -296,22 +276,27 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     return LBM_OK;
 }
 
+template <class ST, bool LIT>
+static lbm_status launch_coll(lbm_ctx* c, int nsteps)
+{
+    switch (c->d.collision) {
+    case LBM_COLLISION_CUM2015: return launch_steps<ST, LBM_COLLISION_CUM2015, LIT>(c, nsteps);
+    case LBM_COLLISION_CUM2017: return launch_steps<ST, LBM_COLLISION_CUM2017, LIT>(c, nsteps);
+    case LBM_COLLISION_COMBINED: return launch_steps<ST, LBM_COLLISION_COMBINED, LIT>(c, nsteps);
+    default: return launch_steps<ST, LBM_COLLISION_SRT, LIT>(c, nsteps);
+    }
+}
+
 template <class ST>
 static lbm_status launch_by_coll(lbm_ctx* c, int nsteps)
 {
-    const bool lit = c->d.arith == LBM_ARITH_LITERAL;
-    if (c->d.collision == LBM_COLLISION_CUM2015)
-        return lit ? launch_steps<ST, LBM_COLLISION_CUM2015, true>(c, nsteps)
-                   : launch_steps<ST, LBM_COLLISION_CUM2015, false>(c, nsteps);
-    return lit ? launch_steps<ST, LBM_COLLISION_SRT, true>(c, nsteps)
-               : launch_steps<ST, LBM_COLLISION_SRT, false>(c, nsteps);
+    return c->d.arith == LBM_ARITH_LITERAL ? launch_coll<ST, true>(c, nsteps) : launch_coll<ST, false>(c, nsteps);
 }
 
 template <class ST>
 static lbm_status launch_by_coll_fast(lbm_ctx* c, int nsteps)
 {
-    if (c->d.collision == LBM_COLLISION_CUM2015) return launch_steps<ST, LBM_COLLISION_CUM2015, false>(c, nsteps);
-    return launch_steps<ST, LBM_COLLISION_SRT, false>(c, nsteps);
+    return launch_coll<ST, false>(c, nsteps);
 }
 
 // Dispatch a functor templated on the storage policy.
@@ -587,7 +572,11 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
                         kind[L.user(ox, oy, oz)] != K_STREAM)
                         K |= 1u << q;
                 }
-                P.node[ip] = (uint64_t)S | ((uint64_t)K << 32) | ((uint64_t)wall[iu] << 63);
+                // the Combined operator's 2017 branch: every non-solid cell whose flag
+                // is not 1 (CollisionCumD3Q27TurbulentCombined.h:1071-1076)
+                const bool c17 = d->collision == LBM_COLLISION_COMBINED && d->flags[iu] != 1;
+                P.node[ip] = (uint64_t)S | ((uint64_t)K << 32) | ((uint64_t)c17 << NW_CUM2017) |
+                             ((uint64_t)wall[iu] << 63);
                 if (kind[iu] == K_INLET) {
                     P.cls[ip] = CL_INLET;
                     P.bcidx[ip] = inlet_of[iu];
@@ -596,7 +585,7 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
                     P.cls[ip] = CL_PUB;
                     P.bcidx[ip] = pub_of[iu];
                     P.flags_needed = true;
-                } else if (S != 0 || K != 0 || wall[iu]) {
+                } else if (S != 0 || K != 0 || wall[iu] || c17) {
                     P.cls[ip] = CL_BND;
                     P.flags_needed = true;
                 } else {
@@ -686,12 +675,13 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     *out = nullptr;
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !d->flags) return LBM_E_ARG;
     if (d->n_bc < 0 || (d->n_bc > 0 && !d->bc)) return LBM_E_ARG;
-    if (d->collision != LBM_COLLISION_SRT && d->collision != LBM_COLLISION_CUM2015) return LBM_E_ARG;
+    if (d->collision < LBM_COLLISION_SRT || d->collision > LBM_COLLISION_COMBINED) return LBM_E_ARG;
     if (d->closure != LBM_CLOSURE_NONE && d->closure != LBM_CLOSURE_SMAGORINSKY && d->closure != LBM_CLOSURE_MLP)
         return LBM_E_ARG;
     // the learned closure runs in the fast FP32 arithmetic
     if (d->closure == LBM_CLOSURE_MLP && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->closure == LBM_CLOSURE_MLP && d->storage == LBM_STORAGE_F64) return LBM_E_UNSUPPORTED;
+    if (d->closure == LBM_CLOSURE_MLP && d->collision > LBM_COLLISION_CUM2015) return LBM_E_UNSUPPORTED;
     if (d->storage < LBM_STORAGE_F64 || d->storage > LBM_STORAGE_BF16DEV_SR) return LBM_E_ARG;
     if (d->arith != LBM_ARITH_LITERAL && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->arith == LBM_ARITH_LITERAL && d->storage >= LBM_STORAGE_F16DEV) return LBM_E_ARG;

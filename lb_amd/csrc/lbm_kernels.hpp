@@ -49,13 +49,18 @@ namespace lbm {
 enum : uint8_t {
     CL_SOLID = 0,
     CL_FLUID = 1,  // streamed, every neighbour streamed too
-    CL_BND = 2,    // streamed, some neighbour solid, inlet or outside (masks in the node word)
+    CL_BND = 2,    // streamed, some neighbour solid, inlet or outside (masks in the node word),
+                   // or a cell the Combined operator collides with its 2017 branch
     CL_INLET = 3,  // inlet node (flag -1): own state in a side buffer
     CL_PUB = 4     // streamed like CL_BND; also publishes rho, u_x and f* for a patch kernel
 };
 
 constexpr int TPB = 256;
+// node word of a non-CL_FLUID cell: bits 0-26 source mask S (populations that
+// do not stream in), bits 32-58 destination mask K (streaming writes dropped),
+// bit 62 the Combined operator's 2017 branch (flag != 1), bit 63 wall-list node
 constexpr uint64_t MASK27 = (1ull << 27) - 1;
+constexpr int NW_CUM2017 = 62;
 
 struct Geo {
     int nx, ny, nz;     // padded lattice
@@ -351,6 +356,12 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             load_pop<ST, ODD>(a.p, a.rest, c, f);
         }
 
+        // the Combined operator reads the node word before colliding: bit 62
+        // marks the cells that take the 2017 branch
+        const uint64_t nw17 =
+            (COLL == LBM_COLLISION_COMBINED && FLAGS && cls != CL_FLUID) ? a.node[c.i] : 0ull;
+        const bool c17 = COLL == LBM_COLLISION_CUM2017 || ((nw17 >> NW_CUM2017) & 1ull);
+
         R rho, ux, uy, uz, omega;
         R P[Q];
         if (LIT) {
@@ -367,10 +378,14 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                                                          : a.omega0;
 #pragma unroll
             for (int q = 0; q < Q; ++q) P[q] = f[q];
-            if (COLL == LBM_COLLISION_CUM2015)
-                cum2015_literal(a.K, P, rho, ux, uy, uz, omega);
-            else
+            if (COLL == LBM_COLLISION_SRT)
                 srt_literal(a.K, P, rho, ux, uy, uz, omega);
+            else if (!c17)
+                cum_literal<CV_2015>(a.K, P, rho, ux, uy, uz, omega);
+            else if (COLL == LBM_COLLISION_CUM2017)
+                cum_literal<CV_2017>(a.K, P, rho, ux, uy, uz, omega);
+            else
+                cum_literal<CV_2017C>(a.K, P, rho, ux, uy, uz, omega);
             if (a.diag) {
                 dm[0] = (double)rho;
                 dm[1] = (double)(rho * ux);
@@ -397,12 +412,16 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 omega = omega_from_coefficient((R)a.ccache[c.i], pin_over_rho(ce, rho), a.tau);
             else
                 omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
-            if (COLL == LBM_COLLISION_CUM2015) {
-                cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
-            } else {
+            if (COLL == LBM_COLLISION_SRT) {
 #pragma unroll
                 for (int q = 0; q < Q; ++q) P[q] = f[q];
                 srt_fast(a.K, P, rho, ux, uy, uz, omega);
+            } else if (!c17) {
+                cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
+            } else if (COLL == LBM_COLLISION_CUM2017) {
+                cum2017_fast<CV_2017>(a.K, f, ce, r.m0, rho, ux, uy, uz, omega, P);
+            } else {
+                cum2017_fast<CV_2017C>(a.K, f, ce, r.m0, rho, ux, uy, uz, omega, P);
             }
             if (a.diag) {
                 dm[0] = (double)r.m0;
@@ -414,7 +433,8 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 
         const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
         // node word of a boundary cell (0 for plain fluid): skip masks and walls
-        const uint64_t nw = (FLAGS && cls != CL_FLUID) ? a.node[c.i] : 0ull;
+        const uint64_t nw =
+            (FLAGS && cls != CL_FLUID) ? (COLL == LBM_COLLISION_COMBINED ? nw17 : a.node[c.i]) : 0ull;
         const uint32_t S = (uint32_t)(nw & MASK27);
         const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
         const bool wall = (nw >> 63) != 0;

@@ -47,19 +47,23 @@ static int orc_nnz(int q) { return (ORC_C[q][0] != 0) + (ORC_C[q][1] != 0) + (OR
 #define REAL double
 #define FN(x) x##_d
 #define ORC_SQRT sqrt
+#define ORC_FABS fabs
 #include "lbm_oracle_impl.h"
 #undef REAL
 #undef FN
 #undef ORC_SQRT
+#undef ORC_FABS
 
 /* ---------------- float instantiation ---------------- */
 #define REAL float
 #define FN(x) x##_f
 #define ORC_SQRT sqrtf
+#define ORC_FABS fabsf
 #include "lbm_oracle_impl.h"
 #undef REAL
 #undef FN
 #undef ORC_SQRT
+#undef ORC_FABS
 
 struct orc {
     int dbl;
@@ -87,6 +91,17 @@ int orc_get_threads(void)
 #endif
 }
 
+/* Cumulant constants: 1.f/3.f, 2.f/3.f, 4.f/3.f and lambda = 0.01f
+ * (CollisionCumD3Q27Turbulent2015.h:280-306, Turbulent2017.h:244-246, 349-362). */
+#define ORC_FILL_CK(K, T)                                                                  \
+    do {                                                                                   \
+        const int f32c_ = (compat & LBM_COMPAT_F32CONST) != 0;                             \
+        (K)->c13 = f32c_ ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);                                \
+        (K)->c23 = f32c_ ? (T)(2.f / 3.f) : (T)(2.0 / 3.0);                                \
+        (K)->c43 = f32c_ ? (T)(4.f / 3.f) : (T)(4.0 / 3.0);                                \
+        (K)->lam = f32c_ ? (T)0.01f : (T)0.01;                                             \
+    } while (0)
+
 /* Weights: D3Q27.h:113-118 writes them as float quotients (8.f/27.f ...),
  * which survive RealType = double (quirk Q2). */
 #define ORC_FILL_CONSTS(S, T)                                                              \
@@ -100,6 +115,7 @@ int orc_get_threads(void)
         }                                                                                  \
         orc_crev(compat, (S)->crev);                                                       \
         (S)->c13 = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);       \
+        ORC_FILL_CK(&(S)->ck, T);                                                          \
         (S)->cs = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / sqrtf(3.f)) : (T)(1.0 / sqrt(3.0)); \
     } while (0)
 
@@ -133,7 +149,7 @@ int orc_create(const lbm_desc* d, orc** out)
 {
     if (!d || !out || !d->flags || d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return LBM_E_ARG;
     if (d->n_bc > 0 && !d->bc) return LBM_E_ARG;
-    if (d->collision != LBM_COLLISION_SRT && d->collision != LBM_COLLISION_CUM2015) return LBM_E_ARG;
+    if (d->collision < LBM_COLLISION_SRT || d->collision > LBM_COLLISION_COMBINED) return LBM_E_ARG;
     if (d->closure != LBM_CLOSURE_NONE && d->closure != LBM_CLOSURE_SMAGORINSKY) return LBM_E_UNSUPPORTED;
     orc* o = (orc*)calloc(1, sizeof(orc));
     if (!o) return LBM_E_OOM;
@@ -261,7 +277,7 @@ int orc_error_quadratic(orc* o, double* err)
 /* ---- single-cell entry points for unit tests ---- */
 
 #define ORC_CELL_CONSTS(T)                                                                  \
-    T w[27]; int crev[27]; T c13;                                                           \
+    T w[27]; int crev[27]; T c13; struct { T c13, c23, c43, lam; } ck;                      \
     for (int q = 0; q < 27; ++q) {                                                          \
         int k = orc_nnz(q);                                                                 \
         w[q] = (compat & LBM_COMPAT_F32CONST)                                               \
@@ -269,12 +285,13 @@ int orc_error_quadratic(orc* o, double* err)
                    : (T)(k == 0 ? 8.0 / 27.0 : k == 1 ? 2.0 / 27.0 : k == 2 ? 1.0 / 54.0 : 1.0 / 216.0); \
     }                                                                                       \
     orc_crev(compat, crev); (void)crev;                                                     \
-    c13 = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);
+    c13 = (compat & LBM_COMPAT_F32CONST) ? (T)(1.f / 3.f) : (T)(1.0 / 3.0);                 \
+    ORC_FILL_CK(&ck, T);
 
 void orc_tables(uint32_t compat, int* c, int* c_rev, double* wout)
 {
     ORC_CELL_CONSTS(double)
-    (void)c13;
+    (void)c13; (void)ck;
     for (int q = 0; q < 27; ++q) {
         if (c) { c[3 * q] = ORC_C[q][0]; c[3 * q + 1] = ORC_C[q][1]; c[3 * q + 2] = ORC_C[q][2]; }
         if (c_rev) c_rev[q] = crev[q];
@@ -282,25 +299,26 @@ void orc_tables(uint32_t compat, int* c, int* c_rev, double* wout)
     }
 }
 
+/* One cell of collision operator `collision`; for LBM_COLLISION_COMBINED,
+ * `flag` is the cell's flag-grid value (1 selects the 2015 branch). */
 void orc_cell_collide(int collision, uint32_t compat, int use_double, const double* f, double rho, double ux,
-                      double uy, double uz, double omega, double* out)
+                      double uy, double uz, double omega, int flag, double* out)
 {
     if (use_double) {
         ORC_CELL_CONSTS(double)
+        (void)c13;
+        ck_d k = {ck.c13, ck.c23, ck.c43, ck.lam};
         double o[27];
-        if (collision == LBM_COLLISION_CUM2015)
-            cum2015_d(f, rho, ux, uy, uz, omega, c13, o);
-        else
-            srt_d(w, f, rho, ux, uy, uz, omega, o);
+        collide_one_d(collision, compat, w, &k, flag, f, rho, ux, uy, uz, omega, o);
         for (int q = 0; q < 27; ++q) out[q] = o[q];
     } else {
         ORC_CELL_CONSTS(float)
+        (void)c13;
+        ck_f k = {ck.c13, ck.c23, ck.c43, ck.lam};
         float fi[27], o[27];
         for (int q = 0; q < 27; ++q) fi[q] = (float)f[q];
-        if (collision == LBM_COLLISION_CUM2015)
-            cum2015_f(fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)omega, c13, o);
-        else
-            srt_f(w, fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)omega, o);
+        collide_one_f(collision, compat, w, &k, flag, fi, (float)rho, (float)ux, (float)uy, (float)uz,
+                      (float)omega, o);
         for (int q = 0; q < 27; ++q) out[q] = o[q];
     }
 }
@@ -310,11 +328,11 @@ double orc_cell_omega_les(uint32_t compat, int use_double, const double* f, doub
 {
     if (use_double) {
         ORC_CELL_CONSTS(double)
-        (void)c13;
+        (void)c13; (void)ck;
         return omega_les_d(w, f, rho, ux, uy, uz, tau, c_les);
     } else {
         ORC_CELL_CONSTS(float)
-        (void)c13;
+        (void)c13; (void)ck;
         float fi[27];
         for (int q = 0; q < 27; ++q) fi[q] = (float)f[q];
         return omega_les_f(w, fi, (float)rho, (float)ux, (float)uy, (float)uz, (float)tau, (float)c_les);
@@ -325,11 +343,11 @@ void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double
 {
     if (use_double) {
         ORC_CELL_CONSTS(double)
-        (void)c13;
+        (void)c13; (void)ck;
         for (int q = 0; q < 27; ++q) out[q] = feq_d(w, q, ux, uy, uz, rho);
     } else {
         ORC_CELL_CONSTS(float)
-        (void)c13;
+        (void)c13; (void)ck;
         for (int q = 0; q < 27; ++q) out[q] = feq_f(w, q, (float)ux, (float)uy, (float)uz, (float)rho);
     }
 }

@@ -46,9 +46,11 @@ void orc_set_threads(int n);
 int orc_get_threads(void);
 
 /* Single-cell operators, for unit tests against closed forms.
- * f and out hold 27 values in reference direction order. */
+ * f and out hold 27 values in reference direction order.  flag is the
+ * cell's flag-grid value; only LBM_COLLISION_COMBINED reads it (1 selects the
+ * 2015 branch, anything else the 2017 branch). */
 void orc_cell_collide(int collision, uint32_t compat, int use_double, const double* f, double rho,
-                      double ux, double uy, double uz, double omega, double* out);
+                      double ux, double uy, double uz, double omega, int flag, double* out);
 double orc_cell_omega_les(uint32_t compat, int use_double, const double* f, double rho, double ux,
                           double uy, double uz, double tau, double c_les);
 void orc_cell_feq(uint32_t compat, int use_double, double rho, double ux, double uy, double uz, double* out);

@@ -9,6 +9,13 @@
  * arithmetic (quirk Q2).  Build with -ffp-contract=off.
  */
 
+/* Constants of the cumulant relaxation: 1/3, 2/3, 4/3 and the limiter
+ * parameter lambda = 0.01 (float literals in the reference: 1.f/3.f, 2.f/3.f,
+ * -4.f/3.f, 0.01f; exact values when the F32CONST quirk is off). */
+typedef struct FN(ck) {
+    REAL c13, c23, c43, lam;
+} FN(ck);
+
 typedef struct FN(state) {
     int nx, ny, nz;
     long N;
@@ -16,6 +23,7 @@ typedef struct FN(state) {
     int collision, closure;
     uint32_t compat;
     REAL tau, nu, CLES, c13, cs;
+    FN(ck) ck;
     REAL w[27];
     int crev[27];
     int* flags;
@@ -61,12 +69,32 @@ static inline void FN(inv)(REAL a0, REAL a1, REAL a2, REAL u, REAL* vz, REAL* vm
     *vp = (a0 * (u * u + u) + a1 * (2.f * u + 1.f) + a2) * 0.5f;
 }
 
-/* Cumulant collision of one cell, literal Geier 2015 as configured in the
- * reference: CollisionCumD3Q27Turbulent2015.h:32-516 (omega1 = local omega,
- * omega2..omega10 = 1, :240-249).  f/out in reference direction order. */
-static void FN(cum2015)(const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL omg, REAL c13,
-                        REAL* out)
+/* Variants of the cumulant operator.  All share the central-moment and
+ * cumulant transforms (identical text in the three reference files) and
+ * differ in the relaxation block. */
+enum {
+    FN(V2015) = 0,      /* CollisionCumD3Q27Turbulent2015.h:32-516                    */
+    FN(V2017) = 1,      /* CollisionCumD3Q27Turbulent2017.h:32-570                    */
+    FN(V2017_COMB) = 2  /* CollisionCumD3Q27TurbulentCombined.h:32-573 (cum2017 lambda) */
+};
+
+/* Cumulant collision of one cell in the reference's operation order.
+ * 2015: omega1 = local omega, omega2..omega10 = 1 (Turbulent2015.h:240-249).
+ * 2017: omega1 = local omega, omega2 = 1, omega3..5 from omega1/omega2 with
+ * the magnitude limiters, omega6..10 = 1, and the A/B fourth-order terms
+ * (Turbulent2017.h:238-370).  Quirks (applied when the flag is set):
+ *  - lim210: the 210+012 and 210-012 limiters read |C_120 +- C_102|
+ *    (Turbulent2017.h:254-255, 264-265); corrected: |C_210 +- C_012|;
+ *  - trace17: the bulk term of D_x u subtracts (rho - 1) in Turbulent2017.h:295
+ *    and (rho + 1) in Combined.h:798 instead of the zeroth moment k_000
+ *    that Turbulent2015.h:253 subtracts (the trace of the raw central second
+ *    moments is k_000 at equilibrium, so only k_000 leaves the rest state
+ *    fixed).
+ * f/out in reference direction order. */
+static void FN(cumulant)(int variant, int lim210, int trace17, const FN(ck) * K_, const REAL* f, REAL rho, REAL ux,
+                         REAL uy, REAL uz, REAL omg, REAL* out)
 {
+    const REAL c13 = K_->c13;
     REAL F[3][3][3], A[3][3][3], B[3][3][3], k[3][3][3];
     int a, b, r, q;
     for (q = 0; q < 27; ++q) F[ORC_C[q][0] + 1][ORC_C[q][1] + 1][ORC_C[q][2] + 1] = f[q];
@@ -111,50 +139,153 @@ static void FN(cum2015)(const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL
                                K(1,1,0) * K(1,1,0) * K(0,0,2)) +
                         2.f * K(2,0,0) * K(0,2,0) * K(0,0,2)) / rho / rho;
 
-    /* relaxation rates (:240-249) */
-    const REAL omega1 = omg;
-    const REAL omega2 = 1.f, omega3 = 1.f, omega4 = 1.f, omega5 = 1.f, omega6 = 1.f;
-    const REAL omega7 = 1.f, omega8 = 1.f, omega9 = 1.f, omega10 = 1.f;
+    REAL Dxu, Dyv, Dzw, Cs_110, Cs_101, Cs_011, Cs_200, Cs_020, Cs_002;
+    REAL Cs_120, Cs_102, Cs_210, Cs_012, Cs_021, Cs_201, Cs_111;
+    REAL Cs_220, Cs_202, Cs_022, Cs_211, Cs_121, Cs_112, Cs_221, Cs_212, Cs_122, Cs_222;
+    if (variant == FN(V2015)) {
+        /* relaxation rates (:240-249) */
+        const REAL omega1 = omg;
+        const REAL omega2 = 1.f, omega3 = 1.f, omega4 = 1.f, omega5 = 1.f, omega6 = 1.f;
+        const REAL omega7 = 1.f, omega8 = 1.f, omega9 = 1.f, omega10 = 1.f;
 
-    /* Eq. G2015 58-60 (:252-259) */
-    const REAL Dxu = -omega1 * 0.5f / rho * (2.f * C_200 - C_020 - C_002) -
-                     omega2 * 0.5f / rho * (C_200 + C_020 + C_002 - k_000);
-    const REAL Dyv = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_020);
-    const REAL Dzw = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_002);
+        /* Eq. G2015 58-60 (:252-259) */
+        Dxu = -omega1 * 0.5f / rho * (2.f * C_200 - C_020 - C_002) -
+              omega2 * 0.5f / rho * (C_200 + C_020 + C_002 - k_000);
+        Dyv = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_020);
+        Dzw = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_002);
 
-    /* Eq. G2015 55-57, 61-73 (:264-316) */
-    const REAL Cs_110 = (1.f - omega1) * C_110;
-    const REAL Cs_101 = (1.f - omega1) * C_101;
-    const REAL Cs_011 = (1.f - omega1) * C_011;
-    const REAL Eq61 = (1.f - omega1) * (C_200 - C_020) -
-                      3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uy * uy * Dyv);
-    const REAL Eq64 = (1.f - omega1) * (C_200 - C_002) -
-                      3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uz * uz * Dzw);
-    const REAL Eq65 = k_000 * omega2 + (1.f - omega2) * (C_200 + C_020 + C_002) -
-                      3.f * rho * (1.f - omega2 / 2.f) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
-    const REAL Cs_200 = c13 * (Eq61 + Eq64 + Eq65);
-    const REAL Cs_020 = c13 * (Eq64 - 2.f * Eq61 + Eq65);
-    const REAL Cs_002 = c13 * (Eq61 - 2.f * Eq64 + Eq65);
-    const REAL Cs_120 = (-C_102 - C_120) * omega3 * 0.5f + (C_102 - C_120) * omega4 * 0.5f + C_120;
-    const REAL Cs_102 = (-C_102 - C_120) * omega3 * 0.5f + (-C_102 + C_120) * omega4 * 0.5f + C_102;
-    const REAL Cs_210 = (-C_012 - C_210) * omega3 * 0.5f + (C_012 - C_210) * omega4 * 0.5f + C_210;
-    const REAL Cs_012 = (-C_012 - C_210) * omega3 * 0.5f + (-C_012 + C_210) * omega4 * 0.5f + C_012;
-    const REAL Cs_021 = (-C_021 - C_201) * omega3 * 0.5f + (-C_021 + C_201) * omega4 * 0.5f + C_021;
-    const REAL Cs_201 = (-C_021 - C_201) * omega3 * 0.5f + (C_021 - C_201) * omega4 * 0.5f + C_201;
-    const REAL Cs_111 = (1.f - omega5) * C_111;
-    const REAL Eq71 = (1.f - omega6) * (C_220 - 2.f * C_202 + C_022);
-    const REAL Eq72 = (1.f - omega6) * (C_220 + C_202 - 2.f * C_022);
-    const REAL Eq73 = (1.f - omega7) * (C_220 + C_202 + C_022);
-    const REAL Cs_220 = c13 * (Eq71 + Eq72 + Eq73);
-    const REAL Cs_202 = c13 * (-Eq71 + Eq73);
-    const REAL Cs_022 = c13 * (-Eq72 + Eq73);
-    const REAL Cs_211 = (1.f - omega8) * C_211;
-    const REAL Cs_121 = (1.f - omega8) * C_121;
-    const REAL Cs_112 = (1.f - omega8) * C_112;
-    const REAL Cs_221 = (1.f - omega9) * C_221;
-    const REAL Cs_212 = (1.f - omega9) * C_212;
-    const REAL Cs_122 = (1.f - omega9) * C_122;
-    const REAL Cs_222 = (1.f - omega10) * C_222;
+        /* Eq. G2015 55-57, 61-73 (:264-316) */
+        Cs_110 = (1.f - omega1) * C_110;
+        Cs_101 = (1.f - omega1) * C_101;
+        Cs_011 = (1.f - omega1) * C_011;
+        const REAL Eq61 = (1.f - omega1) * (C_200 - C_020) -
+                          3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uy * uy * Dyv);
+        const REAL Eq64 = (1.f - omega1) * (C_200 - C_002) -
+                          3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uz * uz * Dzw);
+        const REAL Eq65 = k_000 * omega2 + (1.f - omega2) * (C_200 + C_020 + C_002) -
+                          3.f * rho * (1.f - omega2 / 2.f) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
+        Cs_200 = c13 * (Eq61 + Eq64 + Eq65);
+        Cs_020 = c13 * (Eq64 - 2.f * Eq61 + Eq65);
+        Cs_002 = c13 * (Eq61 - 2.f * Eq64 + Eq65);
+        Cs_120 = (-C_102 - C_120) * omega3 * 0.5f + (C_102 - C_120) * omega4 * 0.5f + C_120;
+        Cs_102 = (-C_102 - C_120) * omega3 * 0.5f + (-C_102 + C_120) * omega4 * 0.5f + C_102;
+        Cs_210 = (-C_012 - C_210) * omega3 * 0.5f + (C_012 - C_210) * omega4 * 0.5f + C_210;
+        Cs_012 = (-C_012 - C_210) * omega3 * 0.5f + (-C_012 + C_210) * omega4 * 0.5f + C_012;
+        Cs_021 = (-C_021 - C_201) * omega3 * 0.5f + (-C_021 + C_201) * omega4 * 0.5f + C_021;
+        Cs_201 = (-C_021 - C_201) * omega3 * 0.5f + (C_021 - C_201) * omega4 * 0.5f + C_201;
+        Cs_111 = (1.f - omega5) * C_111;
+        const REAL Eq71 = (1.f - omega6) * (C_220 - 2.f * C_202 + C_022);
+        const REAL Eq72 = (1.f - omega6) * (C_220 + C_202 - 2.f * C_022);
+        const REAL Eq73 = (1.f - omega7) * (C_220 + C_202 + C_022);
+        Cs_220 = c13 * (Eq71 + Eq72 + Eq73);
+        Cs_202 = c13 * (-Eq71 + Eq73);
+        Cs_022 = c13 * (-Eq72 + Eq73);
+        Cs_211 = (1.f - omega8) * C_211;
+        Cs_121 = (1.f - omega8) * C_121;
+        Cs_112 = (1.f - omega8) * C_112;
+        Cs_221 = (1.f - omega9) * C_221;
+        Cs_212 = (1.f - omega9) * C_212;
+        Cs_122 = (1.f - omega9) * C_122;
+        Cs_222 = (1.f - omega10) * C_222;
+    } else {
+        /* relaxation rates, Geier 2017 (Turbulent2017.h:240-291) */
+        const REAL omega1 = omg;
+        const REAL omega2 = 1.f;
+        const REAL lambda3 = K_->lam, lambda4 = K_->lam, lambda5 = K_->lam;
+        const REAL omega3 = 8.f * (omega1 - 2.f) * (omega2 * (3.f * omega1 - 1.f) - 5.f * omega1) /
+                            (8.f * (5.f - 2.f * omega1) * omega1 + omega2 * (8.f + omega1 * (9.f * omega1 - 26.f)));
+        /* limiters: omega + (1 - omega) |C| / (rho lambda + |C|) */
+        const REAL a210p = lim210 ? C_120 + C_102 : C_210 + C_012;
+        const REAL a210m = lim210 ? C_120 - C_102 : C_210 - C_012;
+        const REAL omega120p102 =
+            omega3 + (1.f - omega3) * ORC_FABS(C_120 + C_102) / (rho * lambda3 + ORC_FABS(C_120 + C_102));
+        const REAL omega210p012 = omega3 + (1.f - omega3) * ORC_FABS(a210p) / (rho * lambda3 + ORC_FABS(a210p));
+        const REAL omega201p021 =
+            omega3 + (1.f - omega3) * ORC_FABS(C_201 + C_021) / (rho * lambda3 + ORC_FABS(C_201 + C_021));
+        const REAL omega4 = 8.f * (omega1 - 2.f) * (omega1 + omega2 * (3.f * omega1 - 7.f)) /
+                            (omega2 * (56.f - 42.f * omega1 + 8.f * omega1 * omega1) - 8.f * omega1);
+        const REAL omega120m102 =
+            omega4 + (1.f - omega4) * ORC_FABS(C_120 - C_102) / (rho * lambda4 + ORC_FABS(C_120 - C_102));
+        const REAL omega210m012 = omega4 + (1.f - omega4) * ORC_FABS(a210m) / (rho * lambda4 + ORC_FABS(a210m));
+        const REAL omega201m021 =
+            omega4 + (1.f - omega4) * ORC_FABS(C_201 - C_021) / (rho * lambda4 + ORC_FABS(C_201 - C_021));
+        const REAL omega5 = 24.f * (omega1 - 2.f) *
+                            (4.f * omega1 * omega1 + omega1 * omega2 * (18.f - 13.f * omega1) +
+                             omega2 * omega2 * (2.f + omega1 * (6.f * omega1 - 11.f))) /
+                            (16.f * omega1 * omega1 * (omega1 - 6.f) -
+                             2.f * omega1 * omega2 * (216.f + 5.f * omega1 * (9.f * omega1 - 46.f)) +
+                             omega2 * omega2 * (omega1 * (3.f * omega1 - 10.f) * (15.f * omega1 - 28.f) - 48.f));
+        const REAL omega111 = omega5 + (1.f - omega5) * ORC_FABS(C_111) / (rho * lambda5 + ORC_FABS(C_111));
+        const REAL omega6 = 1.f, omega7 = 1.f, omega8 = 1.f, omega9 = 1.f, omega10 = 1.f;
+        const REAL A = (4.f * omega1 * omega1 + 2.f * omega1 * omega2 * (omega1 - 6.f) +
+                        omega2 * omega2 * (omega1 * (10.f - 3.f * omega1) - 4.f)) /
+                       (omega1 - omega2) / (omega2 * (2.f + 3.f * omega1) - 8.f * omega1);
+        const REAL B = (4.f * omega1 * omega2 * (9.f * omega1 - 16.f) - 4.f * omega1 * omega1 -
+                        2.f * omega2 * omega2 * (2.f + 9.f * omega1 * (omega1 - 2.f))) /
+                       3.f / (omega1 - omega2) / (omega2 * (2.f + 3.f * omega1) - 8.f * omega1);
+
+        /* Eq. G2015 58-60 with the 2017 bulk term (Turbulent2017.h:294-301,
+         * Combined.h:797-798) and the off-diagonal rates (:304-306) */
+        REAL tr;
+        if (!trace17) tr = k_000;
+        else if (variant == FN(V2017)) tr = (-1 + rho);
+        else tr = -(-1 - rho);
+        Dxu = -omega1 * 0.5f / rho * (2.f * C_200 - C_020 - C_002) - omega2 * 0.5f / rho * (C_200 + C_020 + C_002 - tr);
+        Dyv = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_020);
+        Dzw = Dxu + 3.f * omega1 * 0.5f / rho * (C_200 - C_002);
+        const REAL DxvDyu = -3.f * omega1 / rho * C_110;
+        const REAL DxwDzu = -3.f * omega1 / rho * C_101;
+        const REAL DywDzv = -3.f * omega1 / rho * C_011;
+
+        /* second order as in 2015 (:310-329) */
+        Cs_110 = (1.f - omega1) * C_110;
+        Cs_101 = (1.f - omega1) * C_101;
+        Cs_011 = (1.f - omega1) * C_011;
+        const REAL Eq61 = (1.f - omega1) * (C_200 - C_020) -
+                          3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uy * uy * Dyv);
+        const REAL Eq64 = (1.f - omega1) * (C_200 - C_002) -
+                          3.f * rho * (1.f - omega1 * 0.5f) * (ux * ux * Dxu - uz * uz * Dzw);
+        const REAL Eq65 = k_000 * omega2 + (1.f - omega2) * (C_200 + C_020 + C_002) -
+                          3.f * rho * (1.f - omega2 / 2.f) * (ux * ux * Dxu + uy * uy * Dyv + uz * uz * Dzw);
+        Cs_200 = c13 * (Eq61 + Eq64 + Eq65);
+        Cs_020 = c13 * (Eq64 - 2.f * Eq61 + Eq65);
+        Cs_002 = c13 * (Eq61 - 2.f * Eq64 + Eq65);
+
+        /* third order, Eq. G2017 117-122 and 42 (:331-346) */
+        const REAL Eq117 = (1.f - omega120p102) * (C_120 + C_102);
+        const REAL Eq118 = (1.f - omega210p012) * (C_210 + C_012);
+        const REAL Eq119 = (1.f - omega201p021) * (C_201 + C_021);
+        const REAL Eq120 = (1.f - omega120m102) * (C_120 - C_102);
+        const REAL Eq121 = (1.f - omega210m012) * (C_210 - C_012);
+        const REAL Eq122 = (1.f - omega201m021) * (C_201 - C_021);
+        Cs_120 = 0.5f * (Eq120 + Eq117);
+        Cs_102 = 0.5f * (-Eq120 + Eq117);
+        Cs_210 = 0.5f * (Eq121 + Eq118);
+        Cs_012 = 0.5f * (-Eq121 + Eq118);
+        Cs_021 = 0.5f * (-Eq122 + Eq119);
+        Cs_201 = 0.5f * (Eq122 + Eq119);
+        Cs_111 = (1.f - omega111) * C_111;
+
+        /* fourth order, Eq. G2017 43-48 (:349-365) */
+        const REAL Eq43 = K_->c23 * (1.f / omega1 - 0.5f) * omega6 * A * rho * (Dxu - 2.f * Dyv + Dzw) +
+                          (1.f - omega6) * (C_220 - 2.f * C_202 + C_022);
+        const REAL Eq44 = K_->c23 * (1.f / omega1 - 0.5f) * omega6 * A * rho * (Dxu + Dyv - 2.f * Dzw) +
+                          (1.f - omega6) * (C_220 + C_202 - 2.f * C_022);
+        const REAL Eq45 = -K_->c43 * (1.f / omega1 - 0.5f) * omega7 * A * rho * (Dxu + Dyv + Dzw) +
+                          (1.f - omega7) * (C_220 + C_202 + C_022);
+        Cs_220 = c13 * (Eq43 + Eq44 + Eq45);
+        Cs_202 = c13 * (-Eq43 + Eq45);
+        Cs_022 = c13 * (-Eq44 + Eq45);
+        Cs_211 = -c13 * (1.f / omega1 - 0.5f) * omega8 * B * rho * DywDzv + (1.f - omega8) * C_211;
+        Cs_121 = -c13 * (1.f / omega1 - 0.5f) * omega8 * B * rho * DxwDzu + (1.f - omega8) * C_121;
+        Cs_112 = -c13 * (1.f / omega1 - 0.5f) * omega8 * B * rho * DxvDyu + (1.f - omega8) * C_112;
+
+        /* fifth and sixth order (:367-370) */
+        Cs_221 = (1.f - omega9) * C_221;
+        Cs_212 = (1.f - omega9) * C_212;
+        Cs_122 = (1.f - omega9) * C_122;
+        Cs_222 = (1.f - omega10) * C_222;
+    }
 #undef K
 
     /* back to central moments, Eq. G2015 81-87 (:322-380) */
@@ -280,7 +411,26 @@ static void FN(op_omega)(FN(state) * s)
     }
 }
 
-/* Collision*::collision (Cum2015.h:521-536, CollisionSRTTurbulent.h:56-80) */
+/* One cell of the configured operator.  The Combined operator collides
+ * cells with flag 1 (fluid, no solid neighbour) with the 2015 lambda and every
+ * other non-solid cell with its 2017 lambda
+ * (CollisionCumD3Q27TurbulentCombined.h:1066-1078). */
+static void FN(collide_one)(int collision, uint32_t compat, const REAL* w, const FN(ck) * ck, int flag,
+                            const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL omg, REAL* o)
+{
+    const int lim = (compat & LBM_COMPAT_LIM210) != 0, tr = (compat & LBM_COMPAT_TRACE17) != 0;
+    if (collision == LBM_COLLISION_SRT)
+        FN(srt)(w, f, rho, ux, uy, uz, omg, o);
+    else if (collision == LBM_COLLISION_CUM2015 || (collision == LBM_COLLISION_COMBINED && flag == 1))
+        FN(cumulant)(FN(V2015), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+    else if (collision == LBM_COLLISION_CUM2017)
+        FN(cumulant)(FN(V2017), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+    else
+        FN(cumulant)(FN(V2017_COMB), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+}
+
+/* Collision*::collision (Cum2015.h:521-536, Cum2017.h:575-590, Combined.h:1066-1086,
+ * CollisionSRTTurbulent.h:56-80) */
 static void FN(op_collision)(FN(state) * s)
 {
     long N = s->N;
@@ -289,10 +439,8 @@ static void FN(op_collision)(FN(state) * s)
         if (s->flags[i] == 0) continue;
         REAL f[27], o[27];
         FN(load)(s, s->df, i, f);
-        if (s->collision == LBM_COLLISION_CUM2015)
-            FN(cum2015)(f, s->rho[i], s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->omega[i], s->c13, o);
-        else
-            FN(srt)(s->w, f, s->rho[i], s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], s->omega[i], o);
+        FN(collide_one)(s->collision, s->compat, s->w, &s->ck, s->flags[i], f, s->rho[i], s->u[3 * i],
+                        s->u[3 * i + 1], s->u[3 * i + 2], s->omega[i], o);
         for (int q = 0; q < 27; ++q) DP(s, q, i) = o[q];
     }
 }

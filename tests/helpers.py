@@ -53,7 +53,8 @@ def oracle_lib():
             "orc_set_threads": (None, [ctypes.c_int]),
             "orc_get_threads": (ctypes.c_int, []),
             "orc_cell_collide": (None, [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _dp, ctypes.c_double,
-                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, _dp]),
             "orc_cell_omega_les": (ctypes.c_double, [ctypes.c_uint32, ctypes.c_int, _dp, ctypes.c_double,
                                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                      ctypes.c_double, ctypes.c_double]),

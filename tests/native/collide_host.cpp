@@ -10,15 +10,7 @@ using namespace lbm;
 template <class R>
 static Consts<R> consts(uint32_t compat)
 {
-    Consts<R> K;
-    if (compat & 2u) {
-        K.w[0] = (R)(8.f / 27.f); K.w[1] = (R)(2.f / 27.f); K.w[2] = (R)(1.f / 54.f); K.w[3] = (R)(1.f / 216.f);
-        K.c13 = (R)(1.f / 3.f);
-    } else {
-        K.w[0] = (R)(8.0 / 27.0); K.w[1] = (R)(2.0 / 27.0); K.w[2] = (R)(1.0 / 54.0); K.w[3] = (R)(1.0 / 216.0);
-        K.c13 = (R)(1.0 / 3.0);
-    }
-    return K;
+    return make_consts<R>(compat);
 }
 
 template <class R>
@@ -28,13 +20,23 @@ static void run(int coll, int fast, uint32_t compat, const double* fin, double r
     const Consts<R> K = consts<R>(compat);
     R f[Q];
     for (int q = 0; q < Q; ++q) f[q] = (R)fin[q];
+    // coll: LBM_COLLISION_*; LBM_COLLISION_COMBINED stands for the Combined
+    // operator's 2017 branch (a cell whose flag is not 1)
     if (!fast) {
-        if (coll == 1) cum2015_literal(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+        if (coll == LBM_COLLISION_CUM2015) cum_literal<CV_2015>(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+        else if (coll == LBM_COLLISION_CUM2017) cum_literal<CV_2017>(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
+        else if (coll == LBM_COLLISION_COMBINED) cum_literal<CV_2017C>(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
         else srt_literal(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
     } else {
         const Raw2<R> r = raw_moments(f);
         const Cen2<R> c = central2(r, (R)ux, (R)uy, (R)uz);
-        if (coll == 1) cum2015_fast(K, c, r.m0, (R)rho, (R)ux, (R)uy, (R)uz, (R)om, f);
+        R in[Q];
+        for (int q = 0; q < Q; ++q) in[q] = f[q];
+        if (coll == LBM_COLLISION_CUM2015) cum2015_fast(K, c, r.m0, (R)rho, (R)ux, (R)uy, (R)uz, (R)om, f);
+        else if (coll == LBM_COLLISION_CUM2017)
+            cum2017_fast<CV_2017>(K, in, c, r.m0, (R)rho, (R)ux, (R)uy, (R)uz, (R)om, f);
+        else if (coll == LBM_COLLISION_COMBINED)
+            cum2017_fast<CV_2017C>(K, in, c, r.m0, (R)rho, (R)ux, (R)uy, (R)uz, (R)om, f);
         else srt_fast(K, f, (R)rho, (R)ux, (R)uy, (R)uz, (R)om);
     }
     for (int q = 0; q < Q; ++q) out[q] = (double)f[q];

@@ -77,7 +77,7 @@ def test_collision_conserves_mass_and_momentum(coll, dbl):
         rho = f.sum()
         u = f @ c / rho
         out = np.zeros(27)
-        O.orc_cell_collide(coll, 0, dbl, abi.ptr(f), rho, *u, 1.7, abi.ptr(out))
+        O.orc_cell_collide(coll, 0, dbl, abi.ptr(f), rho, *u, 1.7, 1, abi.ptr(out))
         tol = 1e-14 if dbl else 2e-6
         assert abs(out.sum() - rho) < tol
         assert np.allclose(out @ c, rho * u, atol=tol)
@@ -87,7 +87,7 @@ def test_collision_conserves_mass_and_momentum(coll, dbl):
 def test_rest_equilibrium_is_fixed_point(coll):
     f = feq(1.0, (0.0, 0.0, 0.0))
     out = np.zeros(27)
-    oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), 1.0, 0.0, 0.0, 0.0, 1.9, abi.ptr(out))
+    oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), 1.0, 0.0, 0.0, 0.0, 1.9, 1, abi.ptr(out))
     assert np.allclose(out, f, atol=2e-16)
 
 
@@ -101,7 +101,7 @@ def test_collision_with_omega_one_relaxes_stress_to_equilibrium():
     u = f @ c / rho
     for coll in (abi.COLLISION_SRT, abi.COLLISION_CUM2015):
         out = np.zeros(27)
-        oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), rho, *u, 1.0, abi.ptr(out))
+        oracle_lib().orc_cell_collide(coll, 0, 1, abi.ptr(f), rho, *u, 1.0, 1, abi.ptr(out))
         cc = c - u
         k2 = np.einsum("q,qa,qb->ab", out, cc, cc)
         dev = k2 - np.trace(k2) / 3 * np.eye(3)
@@ -137,7 +137,7 @@ def test_cumulant_relaxes_shear_at_rate_omega():
     u = f @ c / rho
     om = 1.3
     out = np.zeros(27)
-    oracle_lib().orc_cell_collide(abi.COLLISION_CUM2015, 0, 1, abi.ptr(f), rho, *u, om, abi.ptr(out))
+    oracle_lib().orc_cell_collide(abi.COLLISION_CUM2015, 0, 1, abi.ptr(f), rho, *u, om, 1, abi.ptr(out))
     cc = c - u
     k_in = np.einsum("q,qa,qb->ab", f, cc, cc)
     k_out = np.einsum("q,qa,qb->ab", out, cc, cc)
@@ -298,7 +298,7 @@ def test_product_collision_arithmetic_is_bitwise_the_oracle():
             for coll in (0, 1):
                 for compat in (0, abi.COMPAT_REFERENCE):
                     a, b, fast = np.zeros(27), np.zeros(27), np.zeros(27)
-                    O.orc_cell_collide(coll, compat, dbl, abi.ptr(f), *m, 1.9, abi.ptr(a))
+                    O.orc_cell_collide(coll, compat, dbl, abi.ptr(f), *m, 1.9, 1, abi.ptr(a))
                     H.host_collide(coll, 0, dbl, compat, abi.ptr(f), *m, 1.9, abi.ptr(b))
                     H.host_collide(coll, 1, dbl, compat, abi.ptr(f), *m, 1.9, abi.ptr(fast))
                     assert np.array_equal(a, b)

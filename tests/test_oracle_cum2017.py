@@ -1,0 +1,279 @@
+"""Pins of the oracle's Geier-2017 and Combined cumulant operators (no GPU).
+
+The 2017 operator (CollisionCumD3Q27Turbulent2017.h) shares the central-moment
+and cumulant transforms of the 2015 one, which test_oracle.py pins against the
+brute-force definition; here the relaxation block is pinned:
+
+- conservation of mass and momentum, and the rest state as a fixed point;
+- the bulk-term quirk (LBM_COMPAT_TRACE17): the rest state then moves only in
+  the three diagonal fourth-order moments, by equal and opposite amounts in the
+  standalone operator (rho - 1) and the Combined operator's branch (rho + 1);
+- the magnitude limiter of the third-order rates, omega + (1 - omega) |C| /
+  (rho lambda + |C|): the measured post/pre ratio r(C) satisfies
+  r(C) (rho lambda + |C|) = const, which fixes lambda = 0.01; and the 210/012
+  quirk (LBM_COMPAT_LIM210), under which the 210+012 limiter does not see
+  |C_210 + C_012| at all;
+- where every term that separates the two operators vanishes (no third-order
+  moments, no shear, an isotropic stress with the equilibrium trace) the 2017
+  operator equals the 2015 one;
+- the Combined operator collides flag-1 cells with the 2015 operator and other
+  fluid cells with the 2017 one;
+- physics: the optimised omega_3..5 and the B term of Geier 2017 make the
+  viscosity of a shear wave converge faster than the 2015 operator's, whose
+  rates are all 1 (an error in those rational functions loses that).
+The coefficient A(omega_1) enters only through diagonal strain and is pinned
+by restatement alone.
+"""
+import numpy as np
+import pytest
+
+from helpers import OracleLattice, host_collide_lib, oracle_lib, tables
+from lb_amd import abi, make_desc
+
+C17, COMB, C15 = abi.COLLISION_CUM2017, abi.COLLISION_COMBINED, abi.COLLISION_CUM2015
+
+
+def feq(rho, u, compat=0):
+    out = np.zeros(27)
+    oracle_lib().orc_cell_feq(compat, 1, rho, *u, abi.ptr(out))
+    return out
+
+
+def collide(coll, f, rho, u, om, compat=0, flag=2, dbl=1):
+    out = np.zeros(27)
+    oracle_lib().orc_cell_collide(coll, compat, dbl, abi.ptr(np.ascontiguousarray(f, dtype=np.float64)), rho, *u,
+                                  om, flag, abi.ptr(out))
+    return out
+
+
+def central(f, u):
+    k = np.zeros(27)
+    oracle_lib().orc_cell_central_moments(abi.ptr(np.ascontiguousarray(f, dtype=np.float64)), *u, abi.ptr(k))
+    return k
+
+
+def from_central(k, u):
+    f = np.zeros(27)
+    oracle_lib().orc_cell_from_central_moments(abi.ptr(np.ascontiguousarray(k, dtype=np.float64)), *u, abi.ptr(f))
+    return f
+
+
+def ix(i, j, l):
+    return 9 * i + 3 * j + l
+
+
+def rest_moments(rho=1.0):
+    k = np.zeros(27)
+    k[ix(0, 0, 0)] = rho
+    for a in (ix(2, 0, 0), ix(0, 2, 0), ix(0, 0, 2)):
+        k[a] = rho / 3
+    # fourth and sixth order of the rest equilibrium: products of the second
+    k[ix(2, 2, 0)] = k[ix(2, 0, 2)] = k[ix(0, 2, 2)] = rho / 9
+    k[ix(2, 2, 2)] = rho / 27
+    return k
+
+
+@pytest.mark.parametrize("coll", [C17, COMB])
+@pytest.mark.parametrize("compat", [0, abi.COMPAT_REFERENCE], ids=["corrected", "reference"])
+@pytest.mark.parametrize("dbl", [1, 0])
+def test_conserves_mass_and_momentum(coll, compat, dbl):
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        f = feq(1 + 0.02 * rng.standard_normal(), 0.04 * rng.standard_normal(3)) * (1 + 0.03 * rng.standard_normal(27))
+        if not dbl:
+            f = f.astype(np.float32).astype(np.float64)
+        rho = f.sum()
+        u = f @ c / rho
+        out = collide(coll, f, rho, u, 1.85, compat, dbl=dbl)
+        tol = 1e-14 if dbl else 2e-6
+        assert abs(out.sum() - rho) < tol
+        assert np.allclose(out @ c, rho * u, atol=tol)
+
+
+def test_equilibrium_rest_moments_match_the_lattice():
+    """rest_moments() is what the transform gives for the rest equilibrium."""
+    k = central(feq(1.0, (0.0, 0.0, 0.0)), (0.0, 0.0, 0.0))
+    assert np.allclose(k, rest_moments(), atol=1e-16)
+
+
+@pytest.mark.parametrize("coll", [C17, COMB])
+@pytest.mark.parametrize("om", [1.2, 1.85, 1.99])
+def test_rest_state_is_fixed_point(coll, om):
+    f = feq(1.0, (0.0, 0.0, 0.0))
+    for compat in (0, abi.COMPAT_LIM210 | abi.COMPAT_CREV20):
+        out = collide(coll, f, 1.0, (0.0, 0.0, 0.0), om, compat)
+        assert np.abs(out - f).max() < 3e-16
+
+
+@pytest.mark.parametrize("om", [1.2, 1.85, 1.99])
+def test_trace_quirk_moves_only_diagonal_fourth_moments(om):
+    f = feq(1.0, (0.0, 0.0, 0.0))
+    u0 = (0.0, 0.0, 0.0)
+    k_eq = central(f, u0)
+    d17 = central(collide(C17, f, 1.0, u0, om, abi.COMPAT_TRACE17), u0) - k_eq
+    dcb = central(collide(COMB, f, 1.0, u0, om, abi.COMPAT_TRACE17), u0) - k_eq
+    diag4 = [ix(2, 2, 0), ix(2, 0, 2), ix(0, 2, 2)]
+    others = [a for a in range(27) if a not in diag4 and a != ix(2, 2, 2)]
+    # the three diagonal fourth moments move by the same amount delta (and the
+    # sixth by the matching product); everything else stays
+    assert np.abs(d17[others]).max() < 1e-15 and np.abs(dcb[others]).max() < 1e-15
+    delta = d17[diag4[0]]
+    assert abs(delta) > 1e-4
+    assert np.allclose(d17[diag4], delta, rtol=1e-12, atol=0)
+    # rho - 1 and rho + 1 sit symmetrically about the corrected k_000: opposite shifts
+    assert np.allclose(dcb[diag4], -delta, rtol=1e-12, atol=0)
+    # the sixth-order moment follows through the product terms, linear in delta
+    assert abs(d17[ix(2, 2, 2)] + dcb[ix(2, 2, 2)]) < 1e-15
+
+
+def _third_order_state(pairs, u=(0.0, 0.0, 0.0)):
+    k = rest_moments()
+    for (i, j, l), v in pairs.items():
+        k[ix(i, j, l)] = v
+    return from_central(k, u)
+
+
+def _ratio(compat, amp, om=1.9):
+    """Post/pre ratio of C_210 + C_012 for a state carrying only that sum."""
+    f = _third_order_state({(2, 1, 0): amp, (0, 1, 2): amp})
+    u0 = (0.0, 0.0, 0.0)
+    k = central(collide(C17, f, 1.0, u0, om, compat), u0)
+    return (k[ix(2, 1, 0)] + k[ix(0, 1, 2)]) / (2 * amp)
+
+
+def test_limiter_form_and_lambda():
+    amps = [1e-6, 1e-3, 4e-3, 2e-2]
+    r = np.array([_ratio(0, a) for a in amps])
+    x = 2 * np.array(amps)   # |C_210 + C_012|
+    # r (rho lambda + |C|) is the same for every amplitude
+    prod = r * (0.01 + x)
+    assert np.allclose(prod, prod[0], rtol=1e-9)
+    # lambda recovered from two amplitudes
+    lam = (r[3] * x[3] - r[1] * x[1]) / (r[1] - r[3])
+    assert abs(lam - 0.01) < 1e-10
+    # the limit of small |C| is 1 - omega_3 with 0 < omega_3 < 2 (a stable rate)
+    w3 = 1 - r[0]
+    assert 0 < w3 < 2
+    # large |C|: the limiter drives the rate to 1 (the cumulant is erased)
+    assert abs(_ratio(0, 10.0)) < 1e-3
+
+
+def test_lim210_quirk_ignores_the_210_012_magnitude():
+    """With LBM_COMPAT_LIM210 the 210+012 limiter reads |C_120 + C_102|,
+    which is zero here: the ratio no longer depends on the amplitude and equals
+    the unlimited 1 - omega_3."""
+    r = [_ratio(abi.COMPAT_LIM210, a) for a in (1e-6, 1e-3, 2e-2)]
+    assert np.allclose(r, r[0], rtol=1e-9)
+    assert abs(r[0] - _ratio(0, 1e-9)) < 1e-6
+
+
+@pytest.mark.parametrize("om", [1.3, 1.9])
+def test_equals_2015_when_separating_terms_vanish(om):
+    """No third-order moments, no shear, isotropic stress with trace k_000:
+    the limiters, the A and the B terms all multiply zero."""
+    rng = np.random.default_rng(8)
+    k = rest_moments(1.02)
+    for a in (ix(2, 2, 0), ix(2, 0, 2), ix(0, 2, 2), ix(2, 2, 2), ix(2, 1, 1), ix(1, 2, 2), ix(2, 2, 1)):
+        k[a] += 0.01 * rng.standard_normal()
+    u0 = (0.0, 0.0, 0.0)
+    f = from_central(k, u0)
+    a = collide(C17, f, f.sum(), u0, om, 0)
+    b = collide(C15, f, f.sum(), u0, om, 0)
+    assert np.abs(a - b).max() < 1e-15
+    # and the test state is not trivial: the operator changes it
+    assert np.abs(a - f).max() > 1e-3
+
+
+def test_combined_dispatches_on_flag():
+    c, _, _ = tables(0)
+    rng = np.random.default_rng(9)
+    f = feq(1.01, (0.03, -0.02, 0.01)) * (1 + 0.03 * rng.standard_normal(27))
+    rho = f.sum()
+    u = f @ c / rho
+    for compat in (0, abi.COMPAT_REFERENCE):
+        c15 = collide(C15, f, rho, u, 1.9, compat)
+        c17 = collide(C17, f, rho, u, 1.9, compat)
+        assert np.array_equal(collide(COMB, f, rho, u, 1.9, compat, flag=1), c15)
+        for flag in (2, 5, -1, -2, -3):
+            cb = collide(COMB, f, rho, u, 1.9, compat, flag=flag)
+            if compat & abi.COMPAT_TRACE17:
+                assert not np.array_equal(cb, c17)   # rho + 1 instead of rho - 1
+            else:
+                assert np.array_equal(cb, c17)
+        assert not np.array_equal(c15, c17)
+
+
+def test_combined_lattice_with_interior_only_is_the_2015_operator():
+    """A fully periodic lattice has flag 1 everywhere: Combined == 2015."""
+    n = 6
+    rng = np.random.default_rng(2)
+    u = [0.02 * rng.standard_normal((n, n, n)) for _ in range(3)]
+    out = []
+    for coll in (C15, COMB):
+        o = OracleLattice(make_desc(np.ones((n, n, n), np.int32), [], collision=coll, storage=abi.STORAGE_F64,
+                                    arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1),
+                                    tau=0.52))
+        o.init_equilibrium(None, *u)
+        o.step(5)
+        out.append(o.populations())
+    assert np.array_equal(out[0], out[1])
+
+
+def _shear_nu_error(coll, L, tau):
+    """Relative error of the viscosity measured from a decaying shear wave
+    u_y(x) = U sin(2 pi x / L) on a periodic L x 1 x 1 lattice."""
+    shape = (1, 1, L)
+    x = np.arange(L)
+    mode = np.sin(2 * np.pi * x / L)
+    d = make_desc(np.ones(shape, np.int32), [], collision=coll, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+                  compat=0, periodic=(1, 1, 1), tau=tau)
+    o = OracleLattice(d)
+    o.init_equilibrium(None, None, np.broadcast_to(1e-4 * mode, shape).copy(), None)
+    nu = (tau - 0.5) / 3
+    k2 = (2 * np.pi / L) ** 2
+    steps = int(0.3 / (nu * k2))
+    o.step(20)
+
+    def amp():
+        return 2 * np.mean(o.macroscopic()[2][0, 0] * mode)
+
+    a0 = amp()
+    o.step(steps)
+    a1 = amp()
+    return (-np.log(a1 / a0) / (k2 * steps)) / nu - 1
+
+
+def test_shear_wave_viscosity_converges_faster_than_2015():
+    tau = 0.526   # omega_1 = 1.90: omega_3..5 = 0.41, 0.047, 0.16, all in (0, 2)
+    e15 = [abs(_shear_nu_error(C15, L, tau)) for L in (16, 32)]
+    e17 = [abs(_shear_nu_error(C17, L, tau)) for L in (16, 32)]
+    print(f"shear-wave nu error L=16,32: 2015 {e15[0]:.3e} {e15[1]:.3e}; 2017 {e17[0]:.3e} {e17[1]:.3e}")
+    # 2015: second order (error ratio about 4 per halving); 2017: faster
+    assert 3.0 < e15[0] / e15[1] < 5.0
+    assert e17[0] / e17[1] > 6.0
+    assert e17[1] < 0.1 * e15[1]
+
+
+def test_product_collision_arithmetic_is_bitwise_the_oracle_2017():
+    """lb_amd/csrc/collide.hpp compiled for the host: the literal 2017 and
+    Combined-branch operators equal the oracle bit for bit; the reduced
+    variants (omega_2 substituted, order 5 and 6 cumulants dropped) agree to a
+    few ulp."""
+    H = host_collide_lib()
+    rng = np.random.default_rng(12)
+    for it in range(60):
+        f0 = feq(1 + 0.02 * rng.standard_normal(), 0.05 * rng.standard_normal(3)) * (1 + 0.02 * rng.standard_normal(27))
+        om = (1.3, 1.9, 1.99)[it % 3]
+        for dbl in (1, 0):
+            f = f0 if dbl else f0.astype(np.float32).astype(np.float64)
+            m = np.zeros(4)
+            H.host_moments(dbl, abi.ptr(f), abi.ptr(m))
+            for coll in (C17, COMB):
+                for compat in (0, abi.COMPAT_REFERENCE):
+                    a = collide(coll, f, m[0], m[1:], om, compat, dbl=dbl)
+                    b, fast = np.zeros(27), np.zeros(27)
+                    H.host_collide(coll, 0, dbl, compat, abi.ptr(f), *m, om, abi.ptr(b))
+                    H.host_collide(coll, 1, dbl, compat, abi.ptr(f), *m, om, abi.ptr(fast))
+                    assert np.array_equal(a, b), (coll, compat, dbl)
+                    assert np.abs(fast - a).max() < (2e-15 if dbl else 5e-7), (coll, compat, dbl)
