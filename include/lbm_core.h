@@ -190,6 +190,26 @@ lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
  * solid.  (No reference counterpart; the learned closure is excluded.) */
 lbm_status lbm_step_diagnostics(lbm_ctx* ctx, double out[7]);
 
+/* Time averaging, MomentTimeAvg (moments/MomentTimeAvg.h:19-80) as the
+ * solver uses it (SolverTurbulentLES.h:216-221, 255-264).  Running sums of
+ * rho, ux, uy, uz per cell, in FP64, on the device (4 * 8 bytes per cell,
+ * allocated by the first call).  The values added are those
+ * lbm_get_macroscopic would return for the state (0 on solid cells).
+ *  - lbm_moment_add: momentAdd for the current state.  The add itself is done
+ *    by the next lbm_step's kernel, which reads that state anyway (no extra
+ *    pass over the populations); a call that changes or reads the state first
+ *    does it with a pass of its own.
+ *  - lbm_step_moment_add: nsteps steps, counting the state after each one
+ *    (the reference's momentAdd after every momentUpdate of the window).
+ *  - lbm_moment_avg: momentAvg, sums / count into caller arrays (any may be
+ *    NULL; the sums are kept, so averaging can continue); *count = states
+ *    added.  LBM_E_STATE when nothing was added.  Synchronises.
+ *  - lbm_moment_avg_reset: zero the sums and the count. */
+lbm_status lbm_moment_add(lbm_ctx* ctx);
+lbm_status lbm_step_moment_add(lbm_ctx* ctx, int32_t nsteps);
+lbm_status lbm_moment_avg(lbm_ctx* ctx, double* rho, double* ux, double* uy, double* uz, int64_t* count);
+lbm_status lbm_moment_avg_reset(lbm_ctx* ctx);
+
 /* Steps completed so far. */
 lbm_status lbm_step_count(lbm_ctx* ctx, int64_t* steps);
 

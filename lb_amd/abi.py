@@ -67,6 +67,10 @@ CORE_SYMBOLS = [
     ("lbm_get_omega", ctypes.c_int, [_vp, _dp]),
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_diagnostics", ctypes.c_int, [_vp, _dp]),
+    ("lbm_moment_add", ctypes.c_int, [_vp]),
+    ("lbm_step_moment_add", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("lbm_moment_avg", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    ("lbm_moment_avg_reset", ctypes.c_int, [_vp]),
     ("lbm_step_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("lbm_last_kernel_ms", ctypes.c_int, [_vp, _dp]),
     ("lbm_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),

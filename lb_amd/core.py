@@ -91,6 +91,24 @@ class Lattice:
     def sync(self):
         _check(self._lib.lbm_sync(self._h), "lbm_sync")
 
+    def moment_add(self):
+        """MomentTimeAvg::momentAdd of the current state (done by the next step's kernel)."""
+        _check(self._lib.lbm_moment_add(self._h), "lbm_moment_add")
+
+    def step_moment_add(self, n: int = 1):
+        """n steps, adding rho and u of the state after each one to the time average."""
+        _check(self._lib.lbm_step_moment_add(self._h, int(n)), "lbm_step_moment_add")
+
+    def moment_avg(self):
+        """MomentTimeAvg::momentAvg: (rho, ux, uy, uz) averages and the number of states added."""
+        out = [np.empty(self.shape) for _ in range(4)]
+        n = ctypes.c_int64()
+        _check(self._lib.lbm_moment_avg(self._h, *[abi.ptr(a) for a in out], ctypes.byref(n)), "lbm_moment_avg")
+        return out, n.value
+
+    def moment_avg_reset(self):
+        _check(self._lib.lbm_moment_avg_reset(self._h), "lbm_moment_avg_reset")
+
     def macroscopic(self):
         out = [np.empty(self.shape) for _ in range(4)]
         _check(self._lib.lbm_get_macroscopic(self._h, *[abi.ptr(a) for a in out]), "lbm_get_macroscopic")

@@ -68,6 +68,11 @@ struct lbm_ctx {
     float* ccache = nullptr;      // coefficient per cell held between network evaluations
     int cl_every = 1;             // evaluate the network every cl_every steps
     int64_t cl_count = 0;         // steps since the last (re)start of the refresh cycle
+    // time averaging (MomentTimeAvg)
+    double* avg = nullptr;        // 4 FP64 planes, caller layout: sums of rho, ux, uy, uz
+    int64_t avg_count = 0;        // states added
+    bool avg_pending = false;     // the current state is counted but not yet added
+    bool avg_each = false;        // launch_steps: count the state after every step
 };
 
 #define HIPCHK(x)                                   \
@@ -184,6 +189,19 @@ static lbm_status launch_patches(lbm_ctx* c, int odd_next)
     return LBM_OK;
 }
 
+// Add the pending state to the time-average sums with its own pass.
+template <class ST>
+static lbm_status avg_flush_st(lbm_ctx* c)
+{
+    if (!c->avg_pending) return LBM_OK;
+    AuxArgs<ST> a = aux_args<ST>(c);
+    a.io = c->avg;
+    k_moment_add<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
+    HIPCHK(hipGetLastError());
+    c->avg_pending = false;
+    return LBM_OK;
+}
+
 // Seed of the stochastic-rounding stores of step t (any fixed bijective mix
 // of the step counter: runs with the same inputs give the same results).
 static uint32_t step_seed(uint64_t t)
@@ -230,6 +248,14 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         a.first = c->first ? 1 : 0;
+        // a counted state is added by the step that reads it; the network step
+        // does not fuse the add, so it gets its own pass first
+        if (mlp && c->avg_pending && c->cl_count % c->cl_every == 0) {
+            lbm_status st = avg_flush_st<ST>(c);
+            if (st != LBM_OK) return st;
+        }
+        a.avg = c->avg_pending ? c->avg : nullptr;
+        c->avg_pending = false;
         a.diag = (s == 0 && c->diag_next) ? 1 : 0;
         if constexpr (ST::sr) a.seed = step_seed((uint64_t)c->steps);
         const bool odd = (c->steps & 1) != 0;
@@ -271,6 +297,10 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         }
         c->first = false;
         c->steps++;
+        if (c->avg_each) {
+            c->avg_pending = true;
+            c->avg_count++;
+        }
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return LBM_OK;
@@ -660,7 +690,7 @@ void lbm_destroy(lbm_ctx* c)
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
-                    c->cl_wb,    c->ccache};
+                    c->cl_wb,    c->ccache,   c->avg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -810,10 +840,19 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     return LBM_OK;
 }
 
+// Add a pending time-average state before the state changes or the sums are read.
+static lbm_status avg_flush(lbm_ctx* c)
+{
+    if (!c->avg_pending) return LBM_OK;
+    return with_storage(c, [&](auto stv) -> lbm_status { return avg_flush_st<decltype(stv)>(c); });
+}
+
 lbm_status lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* ux, const double* uy, const double* uz)
 {
     if (!c) return LBM_E_ARG;
-    lbm_status st = c->rbytes == 8 ? upload_macro<double>(c, rho, ux, uy, uz)
+    lbm_status st = avg_flush(c);
+    if (st != LBM_OK) return st;
+    st = c->rbytes == 8 ? upload_macro<double>(c, rho, ux, uy, uz)
                                    : upload_macro<float>(c, rho, ux, uy, uz);
     if (st != LBM_OK) return st;
     // solid cells carry rho = u = 0 in the reference (InitializationEquilibriumConstVector.h:84-89)
@@ -838,6 +877,10 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* ux,
 lbm_status lbm_set_populations(lbm_ctx* c, const double* f)
 {
     if (!c || !f) return LBM_E_ARG;
+    {
+        lbm_status st = avg_flush(c);
+        if (st != LBM_OK) return st;
+    }
     for (int q = 0; q < Q; ++q) {
         HIPCHK(hipMemcpyAsync(c->stage, f + (size_t)q * c->NU, c->NU * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
@@ -875,6 +918,72 @@ lbm_status lbm_step(lbm_ctx* c, int32_t nsteps)
     }
     if (st != LBM_OK) return st;
     c->last_ms = -(double)nsteps;  // resolved at the next sync
+    return LBM_OK;
+}
+
+static lbm_status avg_alloc(lbm_ctx* c)
+{
+    if (c->avg) return LBM_OK;
+    lbm_status st = dev_alloc(c, (void**)&c->avg, 4 * c->NU * sizeof(double));
+    if (st != LBM_OK) return st;
+    HIPCHK(hipMemsetAsync(c->avg, 0, 4 * c->NU * sizeof(double), c->stream));
+    c->avg_count = 0;
+    c->avg_pending = false;
+    return LBM_OK;
+}
+
+lbm_status lbm_moment_add(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    lbm_status st = avg_alloc(c);
+    if (st == LBM_OK) st = avg_flush(c);   // the same state counted twice
+    if (st != LBM_OK) return st;
+    c->avg_pending = true;
+    c->avg_count++;
+    return LBM_OK;
+}
+
+lbm_status lbm_step_moment_add(lbm_ctx* c, int32_t nsteps)
+{
+    if (!c || nsteps < 0) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    lbm_status st = avg_alloc(c);
+    if (st != LBM_OK) return st;
+    c->avg_each = true;
+    st = lbm_step(c, nsteps);
+    c->avg_each = false;
+    return st;
+}
+
+lbm_status lbm_moment_avg(lbm_ctx* c, double* rho, double* ux, double* uy, double* uz, int64_t* count)
+{
+    if (!c) return LBM_E_ARG;
+    if (count) *count = c->avg ? c->avg_count : 0;
+    if (!c->avg || c->avg_count == 0) return LBM_E_STATE;
+    lbm_status st = avg_flush(c);
+    if (st != LBM_OK) return st;
+    double* outs[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k])
+            HIPCHK(hipMemcpyAsync(outs[k], c->avg + k * c->NU, c->NU * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // momentAvg (MomentTimeAvg.h:47-77): the sums divided by the counter
+    const double n = (double)c->avg_count;
+    for (int k = 0; k < 4; ++k)
+        if (outs[k])
+            for (size_t i = 0; i < c->NU; ++i) outs[k][i] /= n;
+    return LBM_OK;
+}
+
+lbm_status lbm_moment_avg_reset(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    if (!c->avg) return LBM_OK;
+    HIPCHK(hipMemsetAsync(c->avg, 0, 4 * c->NU * sizeof(double), c->stream));
+    c->avg_count = 0;
+    c->avg_pending = false;
     return LBM_OK;
 }
 

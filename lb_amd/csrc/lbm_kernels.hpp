@@ -101,6 +101,8 @@ struct StepArgs {
     const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
     double* partials;              // [ntiles][8] when diag != 0: mass, momentum, wall force
     float* ccache;                 // learned closure: coefficient per cell, held between refreshes
+    double* avg;                   // non-null: add rho, u of the state this step reads to the
+                                   // time-average sums (4 FP64 planes, caller layout)
     Geo g;
     Consts<typename ST::R> K;
     typename ST::R rest[4];        // w*rho0 per weight class (deviation storage)
@@ -270,6 +272,23 @@ __device__ __forceinline__ R fast_rcp(R x)
     return R(1) / x;
 }
 
+// Caller-layout index of a padded cell.
+__device__ __forceinline__ size_t user_index(const Geo& g, const Cell& c)
+{
+    return (size_t)(c.x - g.px) + (size_t)g.ux * ((size_t)(c.y - g.py) + (size_t)g.uy * (size_t)(c.z - g.pz));
+}
+
+// MomentTimeAvg::momentAdd of one cell (MomentTimeAvg.h:19-45): running FP64
+// sums of rho and u, four planes of nu cells.
+__device__ __forceinline__ void moment_add(double* avg, size_t nu, size_t iu, double rho, double ux, double uy,
+                                           double uz)
+{
+    avg[iu] += rho;
+    avg[nu + iu] += ux;
+    avg[2 * nu + iu] += uy;
+    avg[3 * nu + iu] += uz;
+}
+
 // Streaming, bounce-back and boundary bookkeeping of one cell after its
 // collision: f = pre-collision populations, P = post-collision populations.
 // (k_step keeps its own inline copy of this code: passing the population
@@ -374,6 +393,9 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             } else {
                 moments_literal(f, rho, ux, uy, uz);
             }
+            if (a.avg)
+                moment_add(a.avg, (size_t)a.g.ux * a.g.uy * a.g.uz, user_index(a.g, c), (double)rho, (double)ux,
+                           (double)uy, (double)uz);
             omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
                                                          : a.omega0;
 #pragma unroll
@@ -406,6 +428,13 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 ux = r.jx * ir;
                 uy = r.jy * ir;
                 uz = r.jz * ir;
+            }
+            if (a.avg) {
+                // the values the macroscopic read-back forms (u = j / rho)
+                const bool fs = a.first != 0;
+                moment_add(a.avg, (size_t)a.g.ux * a.g.uy * a.g.uz, user_index(a.g, c), (double)rho,
+                           (double)(fs ? ux : r.jx / rho), (double)(fs ? uy : r.jy / rho),
+                           (double)(fs ? uz : r.jz / rho));
             }
             const Cen2<R> ce = central2(r, ux, uy, uz);
             if (CC)   // learned closure between refreshes: held coefficient, current stress
@@ -928,6 +957,56 @@ __global__ __launch_bounds__(TPB) void k_gather(AuxArgs<ST> a)
     a.io[iu] = (double)ST::load(a.p[pl], ix, a.rest[WCLASS[q]]);
 }
 
+// rho, u (and omega when om != nullptr) of a non-solid cell of the current state.
+template <class ST>
+__device__ __forceinline__ void cell_macro(const AuxArgs<ST>& a, const Cell& c, uint8_t cls, typename ST::R& rho,
+                                           typename ST::R& ux, typename ST::R& uy, typename ST::R& uz,
+                                           typename ST::R* om)
+{
+    using R = typename ST::R;
+    R f[Q];
+    cell_state(a, c, cls, f);
+    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    if (a.lit) {
+        if (a.first) {
+            rho = a.macro[c.i];
+            ux = a.macro[N + c.i];
+            uy = a.macro[2 * N + c.i];
+            uz = a.macro[3 * N + c.i];
+        } else {
+            moments_literal(f, rho, ux, uy, uz);
+        }
+        if (om)
+            *om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
+                                                       : a.omega0;
+    } else {
+        const Raw2<R> r = raw_moments(f);
+        if (a.first) {
+            rho = a.macro[c.i];
+            ux = a.macro[N + c.i];
+            uy = a.macro[2 * N + c.i];
+            uz = a.macro[3 * N + c.i];
+        } else {
+            rho = r.m0;
+            ux = r.jx / rho;
+            uy = r.jy / rho;
+            uz = r.jz / rho;
+        }
+        if (om) {
+            // the rate from the velocity as the step forms it (reciprocal, then products)
+            R vx = ux, vy = uy, vz = uz;
+            if (!a.first) {
+                const R ir = fast_rcp(rho);
+                vx = r.jx * ir;
+                vy = r.jy * ir;
+                vz = r.jz * ir;
+            }
+            const Cen2<R> ce = central2(r, vx, vy, vz);
+            *om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+        }
+    }
+}
+
 // rho, u and omega of the current state into 5 FP64 staging planes.
 template <class ST>
 __global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
@@ -942,50 +1021,30 @@ __global__ __launch_bounds__(TPB) void k_macro(AuxArgs<ST> a)
         for (int k = 0; k < 5; ++k) a.io[k * NU + iu] = 0.0;
         return;
     }
-    R f[Q];
-    cell_state(a, c, cls, f);
     R rho, ux, uy, uz, om;
-    const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    if (a.lit) {
-        if (a.first) {
-            rho = a.macro[c.i];
-            ux = a.macro[N + c.i];
-            uy = a.macro[2 * N + c.i];
-            uz = a.macro[3 * N + c.i];
-        } else {
-            moments_literal(f, rho, ux, uy, uz);
-        }
-        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
-                                                  : a.omega0;
-    } else {
-        const Raw2<R> r = raw_moments(f);
-        if (a.first) {
-            rho = a.macro[c.i];
-            ux = a.macro[N + c.i];
-            uy = a.macro[2 * N + c.i];
-            uz = a.macro[3 * N + c.i];
-        } else {
-            rho = r.m0;
-            ux = r.jx / rho;
-            uy = r.jy / rho;
-            uz = r.jz / rho;
-        }
-        // the rate from the velocity as the step forms it (reciprocal, then products)
-        R vx = ux, vy = uy, vz = uz;
-        if (!a.first) {
-            const R ir = fast_rcp(rho);
-            vx = r.jx * ir;
-            vy = r.jy * ir;
-            vz = r.jz * ir;
-        }
-        const Cen2<R> ce = central2(r, vx, vy, vz);
-        om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
-    }
+    cell_macro(a, c, cls, rho, ux, uy, uz, &om);
     a.io[iu] = (double)rho;
     a.io[NU + iu] = (double)ux;
     a.io[2 * NU + iu] = (double)uy;
     a.io[3 * NU + iu] = (double)uz;
     a.io[4 * NU + iu] = (double)om;
+}
+
+// MomentTimeAvg::momentAdd as its own pass: rho, u of the current state (the
+// values k_macro returns) added to the sums in a.io.  A step launched with
+// StepArgs::avg does the same inside the step instead.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_moment_add(AuxArgs<ST> a)
+{
+    using R = typename ST::R;
+    uint32_t iu;
+    Cell c;
+    if (!user_cell(a.g, iu, c)) return;
+    const uint8_t cls = a.cls[c.i];
+    if (cls == CL_SOLID) return;
+    R rho, ux, uy, uz;
+    cell_macro(a, c, cls, rho, ux, uy, uz, (R*)nullptr);
+    moment_add(a.io, (size_t)a.g.ux * a.g.uy * a.g.uz, iu, (double)rho, (double)ux, (double)uy, (double)uz);
 }
 
 // rho, u of a single cell (probe), caller coordinates.

@@ -70,6 +70,8 @@ struct orc {
     double rho0;
     state_d d;
     state_f f;
+    double* avg;          /* time-average sums: rho, ux, uy, uz planes */
+    long long avg_count;
 };
 
 static int g_threads = 0;
@@ -176,6 +178,7 @@ void orc_destroy(orc* o)
     if (!o) return;
     ORC_FREE_STATE(&o->d);
     ORC_FREE_STATE(&o->f);
+    free(o->avg);
     free(o);
 }
 
@@ -228,6 +231,51 @@ int orc_get_macroscopic(orc* o, double* rho, double* ux, double* uy, double* uz)
         ORC_COPY_OUT(&o->d);
     else
         ORC_COPY_OUT(&o->f);
+    return LBM_OK;
+}
+
+/* MomentTimeAvg::momentAdd (MomentTimeAvg.h:19-45): every cell's rho and u
+ * (the fields of the last moment update) added to the sums; the sums are
+ * double here whatever the state's precision. */
+#define ORC_MOMENT_ADD(S)                                                                   \
+    do {                                                                                    \
+        const long N_ = (S)->N;                                                             \
+        for (long i = 0; i < N_; ++i) {                                                     \
+            o->avg[i] += (double)(S)->rho[i];                                               \
+            o->avg[N_ + i] += (double)(S)->u[3 * i];                                        \
+            o->avg[2 * N_ + i] += (double)(S)->u[3 * i + 1];                                \
+            o->avg[3 * N_ + i] += (double)(S)->u[3 * i + 2];                                \
+        }                                                                                   \
+    } while (0)
+
+int orc_moment_add(orc* o)
+{
+    if (!o) return LBM_E_ARG;
+    const long N = o->dbl ? o->d.N : o->f.N;
+    if (!o->avg) {
+        o->avg = (double*)calloc((size_t)4 * N, sizeof(double));
+        if (!o->avg) return LBM_E_OOM;
+        o->avg_count = 0;
+    }
+    if (o->dbl)
+        ORC_MOMENT_ADD(&o->d);
+    else
+        ORC_MOMENT_ADD(&o->f);
+    o->avg_count++;
+    return LBM_OK;
+}
+
+/* MomentTimeAvg::momentAvg (MomentTimeAvg.h:47-77): sums / counter. */
+int orc_moment_avg(orc* o, double* rho, double* ux, double* uy, double* uz, long long* count)
+{
+    if (!o) return LBM_E_ARG;
+    if (count) *count = o->avg ? o->avg_count : 0;
+    if (!o->avg || o->avg_count == 0) return LBM_E_STATE;
+    const long N = o->dbl ? o->d.N : o->f.N;
+    double* outs[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4; ++k)
+        if (outs[k])
+            for (long i = 0; i < N; ++i) outs[k][i] = o->avg[k * N + i] / (double)o->avg_count;
     return LBM_OK;
 }
 

@@ -36,6 +36,9 @@ int orc_init_equilibrium(orc* o, const double* rho, const double* ux, const doub
 int orc_set_populations(orc* o, const double* f);
 int orc_step(orc* o, int nsteps);
 int orc_get_macroscopic(orc* o, double* rho, double* ux, double* uy, double* uz);
+/* MomentTimeAvg: add the current rho, u to FP64 sums; sums / count. */
+int orc_moment_add(orc* o);
+int orc_moment_avg(orc* o, double* rho, double* ux, double* uy, double* uz, long long* count);
 int orc_get_populations(orc* o, double* f);
 int orc_get_post_populations(orc* o, double* f);
 int orc_get_omega(orc* o, double* omega);

@@ -45,6 +45,8 @@ def oracle_lib():
             "orc_set_populations": (ctypes.c_int, [_vp, _dp]),
             "orc_step": (ctypes.c_int, [_vp, ctypes.c_int]),
             "orc_get_macroscopic": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+            "orc_moment_add": (ctypes.c_int, [_vp]),
+            "orc_moment_avg": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
             "orc_get_populations": (ctypes.c_int, [_vp, _dp]),
             "orc_get_post_populations": (ctypes.c_int, [_vp, _dp]),
             "orc_get_omega": (ctypes.c_int, [_vp, _dp]),
@@ -149,6 +151,20 @@ class OracleLattice:
         f = np.empty((27, *self.shape))
         assert self._lib.orc_get_populations(self._h, abi.ptr(f)) == 0
         return f
+
+    def moment_add(self):
+        assert self._lib.orc_moment_add(self._h) == 0
+
+    def step_moment_add(self, n=1):
+        for _ in range(int(n)):
+            self.step(1)
+            self.moment_add()
+
+    def moment_avg(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        n = ctypes.c_longlong()
+        assert self._lib.orc_moment_avg(self._h, *[abi.ptr(a) for a in out], ctypes.byref(n)) == 0
+        return out, n.value
 
     def post_populations(self):
         f = np.empty((27, *self.shape))

@@ -412,3 +412,40 @@ def test_outlet_resets_density_and_inlet_holds_equilibrium():
             got = f[:, z, y, x]
             assert abs(got.sum() - rb) < 1e-13
             assert np.abs(got - feq(rb, (ub, 0.0, 0.0))).max() < 1e-15
+
+
+# --- time averaging -------------------------------------------------------
+
+def test_moment_average_is_the_mean_of_the_snapshots():
+    """MomentTimeAvg (MomentTimeAvg.h:19-77): the average equals the brute-force
+    mean of the rho, u fields read after every counted step, summed in the same
+    order; the count is the number of adds."""
+    from oracle import mesher_oracle as mo
+    from helpers import bc_nodes
+    flags, bcd = mo.channel(16, 8, 6)
+    bc = bc_nodes(bcd)
+    o = OracleLattice(make_desc(flags, bc, collision=abi.COLLISION_COMBINED, storage=abi.STORAGE_F64,
+                                arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=0.53, nu=0.01))
+    rng = np.random.default_rng(4)
+    o.init_equilibrium(None, np.where(flags == 0, 0.0, 0.04 + 0.003 * rng.standard_normal(flags.shape)), None, None)
+    o.step(3)
+    acc = [np.zeros(flags.shape) for _ in range(4)]
+    for k in range(6):
+        o.step(1)
+        o.moment_add()
+        for a, m in zip(acc, o.macroscopic()):
+            a += m
+    avg, n = o.moment_avg()
+    assert n == 6
+    for a, b in zip(avg, acc):
+        assert np.array_equal(a, b / 6)
+    # a uniform steady flow averages to itself
+    flags2 = np.ones((4, 4, 4), np.int32)
+    o2 = OracleLattice(make_desc(flags2, [], collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
+                                 arith=abi.ARITH_LITERAL, compat=0, periodic=(1, 1, 1), tau=0.6))
+    o2.init_equilibrium(None, 0.03, -0.01, 0.02)
+    o2.step_moment_add(5)
+    avg2, n2 = o2.moment_avg()
+    assert n2 == 5
+    assert np.allclose(avg2[0], 1.0, atol=1e-14) and np.allclose(avg2[1], 0.03, atol=1e-15)
+    assert np.allclose(avg2[2], -0.01, atol=1e-15) and np.allclose(avg2[3], 0.02, atol=1e-15)
