@@ -169,6 +169,16 @@ lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
  * (SolverTurbulentLES.h:150-177).  Synchronises. */
 lbm_status lbm_probe(lbm_ctx* ctx, int32_t x, int32_t y, int32_t z, double out[4]);
 
+/* The same four values recorded on the device without synchronising, for a
+ * probe written every step (probe_every_it = 1 makes the reference copy to
+ * the host and wait every step, SolverTurbulentLES.h:150-177, 223-228).
+ * lbm_probe_fetch returns the records made since the last fetch, oldest
+ * first, as {rho, ux, uy, uz} quadruples: *n = their number; out may be NULL
+ * to ask for the count only (nothing is cleared); otherwise max_records must
+ * be >= *n and the records are cleared.  Synchronises. */
+lbm_status lbm_probe_record(lbm_ctx* ctx, int32_t x, int32_t y, int32_t z);
+lbm_status lbm_probe_fetch(lbm_ctx* ctx, double* out, int64_t max_records, int64_t* n);
+
 /* Relaxation rate omega that the next step will use on every cell, computed
  * from the current state as Turbulence::omega does at the start of a step
  * (OmegaLES.h:69-131 / OmegaNo.h:34-47); 0 on solid cells.  Synchronises. */

@@ -97,6 +97,12 @@ int lbmh_case_flags(lbmh_case* c, int32_t* flags);
 lbm_ctx* lbmh_case_lattice(lbmh_case* c);
 /* Error norms recorded by the loop (ErrorQuadratic), oldest first. */
 int lbmh_case_errors(lbmh_case* c, double* out, int32_t* n);
+/* Time averages of rho, ux, uy, uz (lattice units, x fastest) that the run
+ * formed over its last iterationsMomentAvg - 1 steps (MomentTimeAvg,
+ * SolverTurbulentLES.h:216-264; also written to variablesTimeAvg.vtk when
+ * output is on); *count = states averaged.  Non-zero when the run averaged
+ * nothing. */
+int lbmh_case_time_avg(lbmh_case* c, double* rho, double* ux, double* uy, double* uz, int64_t* count);
 void lbmh_case_destroy(lbmh_case* c);
 
 #ifdef __cplusplus

@@ -64,6 +64,8 @@ CORE_SYMBOLS = [
     ("lbm_get_macroscopic", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
     ("lbm_get_populations", ctypes.c_int, [_vp, _dp]),
     ("lbm_probe", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp]),
+    ("lbm_probe_record", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm_probe_fetch", ctypes.c_int, [_vp, _dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     ("lbm_get_omega", ctypes.c_int, [_vp, _dp]),
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_diagnostics", ctypes.c_int, [_vp, _dp]),

@@ -91,6 +91,24 @@ class Lattice:
     def sync(self):
         _check(self._lib.lbm_sync(self._h), "lbm_sync")
 
+    def probe(self, x: int, y: int, z: int):
+        """(rho, ux, uy, uz) of one cell (caller coordinates x, y, z); synchronises."""
+        out = np.empty(4)
+        _check(self._lib.lbm_probe(self._h, int(x), int(y), int(z), abi.ptr(out)), "lbm_probe")
+        return out
+
+    def probe_record(self, x: int, y: int, z: int):
+        """Record (rho, ux, uy, uz) of one cell on the device, without synchronising."""
+        _check(self._lib.lbm_probe_record(self._h, int(x), int(y), int(z)), "lbm_probe_record")
+
+    def probe_fetch(self):
+        """The records made since the last fetch, shape (n, 4), oldest first."""
+        n = ctypes.c_int64()
+        _check(self._lib.lbm_probe_fetch(self._h, None, 0, ctypes.byref(n)), "lbm_probe_fetch")
+        out = np.empty((n.value, 4))
+        _check(self._lib.lbm_probe_fetch(self._h, abi.ptr(out), n.value, ctypes.byref(n)), "lbm_probe_fetch")
+        return out
+
     def moment_add(self):
         """MomentTimeAvg::momentAdd of the current state (done by the next step's kernel)."""
         _check(self._lib.lbm_moment_add(self._h), "lbm_moment_add")

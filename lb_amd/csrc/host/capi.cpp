@@ -1,4 +1,5 @@
 // capi.cpp -- C entry points of liblbm_host.so (include/lbm_host.h).
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <string>
@@ -333,6 +334,19 @@ int lbmh_case_flags(lbmh_case* c, int32_t* flags)
 }
 
 lbm_ctx* lbmh_case_lattice(lbmh_case* c) { return c ? c->D.ctx : nullptr; }
+
+int lbmh_case_time_avg(lbmh_case* c, double* rho, double* ux, double* uy, double* uz, int64_t* count)
+{
+    if (!c) return 1;
+    if (count) *count = c->D.time_avg_count;
+    const size_t N = c->D.rho_avg.size();
+    if (N == 0) return 1;
+    const std::vector<double>* src[4] = {&c->D.rho_avg, &c->D.ux_avg, &c->D.uy_avg, &c->D.uz_avg};
+    double* dst[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) std::copy(src[k]->begin(), src[k]->end(), dst[k]);
+    return 0;
+}
 
 int lbmh_case_errors(lbmh_case* c, double* out, int32_t* n)
 {

@@ -115,6 +115,9 @@ struct LBMData {
     int nx = 0, ny = 0, nz = 0;
     lbm_ctx* ctx = nullptr;  // device lattice (df/df_post/rho/u/omega of the reference)
     std::vector<double> rho_out, ux_out, uy_out, uz_out, omega_out;
+    // rhoTimeAvg / uTimeAvg after momentAvg (MomentTimeAvg.h:47-77), lattice units
+    std::vector<double> rho_avg, ux_avg, uy_avg, uz_avg;
+    int64_t time_avg_count = 0;
 
     void setSizes(int x, int y, int z)
     {
@@ -218,6 +221,8 @@ struct outputerVTK {
     static bool variablesVTK(LBMData& Data, const LBMConstants& Constants, int step, bool verbose);
     static bool variablesLatticeVTK(LBMData& Data, const LBMConstants& Constants, int step, bool verbose);
     static bool omegaVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
+    // variablesTimeAvgVTK (outputerVTK.h:281-345): the time-averaged rho, p, u
+    static bool variablesTimeAvgVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
     static bool probeWriteOut(const LBMConstants& Constants, RealType velX, RealType velY, RealType velZ,
                               RealType rho);
 };
@@ -250,6 +255,7 @@ public:
     int runSimulation();
     void initializeSimulationData(bool verbose);
     int probeWrite();
+    int probeFlush();
     // ErrorQuadratic::errorEvaluation (errorEvaluations/ErrorQuadratic.h:33-90)
     int errorEvaluation();
     bool write_output = true;  // VTK at plot_every_it

@@ -158,6 +158,24 @@ bool outputerVTK::variablesLatticeVTK(LBMData& Data, const LBMConstants& C, int 
                       "pressure");
 }
 
+bool outputerVTK::variablesTimeAvgVTK(LBMData& Data, const LBMConstants& C, bool verbose)
+{
+    if (verbose) std::cout << "\nWriting time averaged moments\n";
+    const size_t N = Data.meshFluidHost.size();
+    if (Data.rho_avg.size() != N) return false;
+    std::ofstream out(C.results_dir + "variablesTimeAvg.vtk");
+    if (!out.is_open()) return false;
+    vtk_header(out, C, "variables");
+    out << "SCALARS rho double 1\nLOOKUP_TABLE default\n";
+    for (double r : Data.rho_avg) out << r * C.Crho << "\n";
+    out << "SCALARS pressure double 1\nLOOKUP_TABLE default\n";
+    for (double r : Data.rho_avg) out << r * C.cs2 * C.Cpressure << "\n";
+    out << "VECTORS U double\n";
+    for (size_t i = 0; i < N; ++i)
+        out << Data.ux_avg[i] * C.Cu << " " << Data.uy_avg[i] * C.Cu << " " << Data.uz_avg[i] * C.Cu << "\n";
+    return (bool)out;
+}
+
 bool outputerVTK::omegaVTK(LBMData& Data, const LBMConstants& C, bool verbose)
 {
     if (verbose) std::cout << "\nDevice -> Host output omega\n";

@@ -154,13 +154,26 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
     return st;
 }
 
+// probeWrite (:150-177): the probe cell's rho, u are recorded on the device
+// without waiting; probeFlush writes the records, oldest first, at the next
+// point where the loop waits for the device anyway.
 int SolverTurbulentLES::probeWrite()
 {
-    double v[4];
-    const int st = lbm_probe(Data.ctx, Constants.ProbeLocationLat[0], Constants.ProbeLocationLat[1],
-                             Constants.ProbeLocationLat[2], v);
+    return lbm_probe_record(Data.ctx, Constants.ProbeLocationLat[0], Constants.ProbeLocationLat[1],
+                            Constants.ProbeLocationLat[2]);
+}
+
+int SolverTurbulentLES::probeFlush()
+{
+    int64_t n = 0;
+    int st = lbm_probe_fetch(Data.ctx, nullptr, 0, &n);
+    if (st != LBM_OK || n == 0) return st;
+    std::vector<double> v(4 * (size_t)n);
+    st = lbm_probe_fetch(Data.ctx, v.data(), n, &n);
     if (st != LBM_OK) return st;
-    outputerVTK::probeWriteOut(Constants, (RealType)v[1], (RealType)v[2], (RealType)v[3], (RealType)v[0]);
+    for (int64_t i = 0; i < n; ++i)
+        outputerVTK::probeWriteOut(Constants, (RealType)v[4 * i + 1], (RealType)v[4 * i + 2], (RealType)v[4 * i + 3],
+                                   (RealType)v[4 * i]);
     return LBM_OK;
 }
 
@@ -209,14 +222,22 @@ int SolverTurbulentLES::runSimulation()
         const bool plot = write_output && Constants.plot_every_it > 0 && kk % Constants.plot_every_it == 0 && kk;
         return probe || err || plot;
     };
+    // time-averaging window: momentAdd after the moment update of every step
+    // k > iterations - iterationsMomentAvg (:216-221); a batch of steps never
+    // straddles its start, so each batch is either counted or not
+    const int avg_from = iters - Constants.iterationsMomentAvg;
     int status = LBM_OK;
     while (k < iters) {
         // advance to the end of step k, or further while no host event is due
         int kk = k;
-        while (kk + 1 < iters && !is_event(kk)) ++kk;
+        while (kk + 1 < iters && !is_event(kk) && kk != avg_from) ++kk;
         timer_collision.start();
-        status = lbm_step(Data.ctx, kk - k + 1);
-        if (status == LBM_OK) status = lbm_sync(Data.ctx);
+        status = k > avg_from ? lbm_step_moment_add(Data.ctx, kk - k + 1) : lbm_step(Data.ctx, kk - k + 1);
+        // wait for the device only where the host reads fields (error norm,
+        // output) and at the end; probe records need no wait
+        const bool err_due = Constants.err_every_it > 0 && kk % Constants.err_every_it == 0 && kk != 0;
+        const bool plot_due = write_output && Constants.plot_every_it > 0 && kk % Constants.plot_every_it == 0 && kk;
+        if (status == LBM_OK && (err_due || plot_due || kk + 1 == iters)) status = lbm_sync(Data.ctx);
         timer_collision.stop();
         if (status != LBM_OK) break;
         k = kk;
@@ -224,6 +245,12 @@ int SolverTurbulentLES::runSimulation()
             k >= (iters - Constants.probe_iterations)) {
             timer_output.start();
             status = probeWrite();
+            timer_output.stop();
+            if (status != LBM_OK) break;
+        }
+        if (err_due || plot_due) {
+            timer_output.start();
+            status = probeFlush();
             timer_output.stop();
             if (status != LBM_OK) break;
         }
@@ -244,6 +271,30 @@ int SolverTurbulentLES::runSimulation()
             timer_output.stop();
         }
         k++;
+    }
+    if (status == LBM_OK || status == LBM_E_NAN) {
+        const int st = probeFlush();
+        if (status == LBM_OK) status = st;
+    }
+    // momentAvg and its output (:255-264), when the window counted any state
+    int64_t counted = 0;
+    if (status == LBM_OK && Constants.iterationsMomentAvg > 0) {
+        timer_timeAvg.start();
+        const size_t N = Data.meshFluidHost.size();
+        Data.rho_avg.resize(N);
+        Data.ux_avg.resize(N);
+        Data.uy_avg.resize(N);
+        Data.uz_avg.resize(N);
+        const int st = lbm_moment_avg(Data.ctx, Data.rho_avg.data(), Data.ux_avg.data(), Data.uy_avg.data(),
+                                      Data.uz_avg.data(), &counted);
+        timer_timeAvg.stop();
+        Data.time_avg_count = counted;
+        if (st != LBM_OK && st != LBM_E_STATE) status = st;
+        if (st == LBM_OK && write_output) {
+            timer_output.start();
+            outputerVTK::variablesTimeAvgVTK(Data, Constants, true);
+            timer_output.stop();
+        }
     }
     timer_loop.stop();
     return status;

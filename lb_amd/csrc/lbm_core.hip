@@ -73,7 +73,13 @@ struct lbm_ctx {
     int64_t avg_count = 0;        // states added
     bool avg_pending = false;     // the current state is counted but not yet added
     bool avg_each = false;        // launch_steps: count the state after every step
+    // probe records taken without a synchronisation
+    double* probe_buf = nullptr;  // [PROBE_CAP][4] on the device
+    int probe_n = 0;              // records in probe_buf
+    std::vector<double> probe_host;  // records drained from the device, oldest first
 };
+
+static constexpr int PROBE_CAP = 4096;
 
 #define HIPCHK(x)                                   \
     do {                                            \
@@ -690,7 +696,7 @@ void lbm_destroy(lbm_ctx* c)
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
-                    c->cl_wb,    c->ccache,   c->avg};
+                    c->cl_wb,    c->ccache,   c->avg,   c->probe_buf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1071,6 +1077,60 @@ lbm_status lbm_probe(lbm_ctx* c, int32_t x, int32_t y, int32_t z, double out[4])
     if (st != LBM_OK) return st;
     HIPCHK(hipMemcpyAsync(out, c->stage, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+// Move the device probe records to the host list (synchronises).
+static lbm_status probe_drain(lbm_ctx* c)
+{
+    if (c->probe_n == 0) return LBM_OK;
+    const size_t old = c->probe_host.size();
+    c->probe_host.resize(old + 4 * (size_t)c->probe_n);
+    HIPCHK(hipMemcpyAsync(c->probe_host.data() + old, c->probe_buf, 4 * (size_t)c->probe_n * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->probe_n = 0;
+    return LBM_OK;
+}
+
+lbm_status lbm_probe_record(lbm_ctx* c, int32_t x, int32_t y, int32_t z)
+{
+    if (!c) return LBM_E_ARG;
+    if (x < 0 || y < 0 || z < 0 || x >= c->g.ux || y >= c->g.uy || z >= c->g.uz) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    if (!c->probe_buf) {
+        lbm_status st = dev_alloc(c, (void**)&c->probe_buf, (size_t)PROBE_CAP * 4 * sizeof(double));
+        if (st != LBM_OK) return st;
+    }
+    if (c->probe_n == PROBE_CAP) {
+        lbm_status st = probe_drain(c);
+        if (st != LBM_OK) return st;
+    }
+    double* slot = c->probe_buf + 4 * (size_t)c->probe_n;
+    lbm_status st = with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        a.io = slot;
+        k_probe<ST><<<1, 64, 0, c->stream>>>(a, x, y, z);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (st != LBM_OK) return st;
+    c->probe_n++;
+    return LBM_OK;
+}
+
+lbm_status lbm_probe_fetch(lbm_ctx* c, double* out, int64_t max_records, int64_t* n)
+{
+    if (!c || !n) return LBM_E_ARG;
+    lbm_status st = probe_drain(c);
+    if (st != LBM_OK) return st;
+    const int64_t have = (int64_t)(c->probe_host.size() / 4);
+    *n = have;
+    if (!out) return LBM_OK;
+    if (max_records < have) return LBM_E_ARG;
+    std::copy(c->probe_host.begin(), c->probe_host.end(), out);
+    c->probe_host.clear();
     return LBM_OK;
 }
 

@@ -1047,7 +1047,8 @@ __global__ __launch_bounds__(TPB) void k_moment_add(AuxArgs<ST> a)
     moment_add(a.io, (size_t)a.g.ux * a.g.uy * a.g.uz, iu, (double)rho, (double)ux, (double)uy, (double)uz);
 }
 
-// rho, u of a single cell (probe), caller coordinates.
+// rho, u of a single cell (probe), caller coordinates, as k_macro forms them;
+// written to a.io[0..3].
 template <class ST>
 __global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
 {
@@ -1060,17 +1061,7 @@ __global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
         return;
     }
     R rho, ux, uy, uz;
-    if (a.first) {
-        const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-        rho = a.macro[c.i];
-        ux = a.macro[N + c.i];
-        uy = a.macro[2 * N + c.i];
-        uz = a.macro[3 * N + c.i];
-    } else {
-        R f[Q];
-        cell_state(a, c, cls, f);
-        moments_literal(f, rho, ux, uy, uz);
-    }
+    cell_macro(a, c, cls, rho, ux, uy, uz, (R*)nullptr);
     a.io[0] = (double)rho;
     a.io[1] = (double)ux;
     a.io[2] = (double)uy;

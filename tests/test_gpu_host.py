@@ -26,7 +26,9 @@ def test_case_cavity_run_matches_oracle(gpu, tmp_path):
                                     (ctypes.c_float * 3)(0, 0, 0), (ctypes.c_float * 3)(n, n, n)) == 0
         for k, v in (("tau", tau), ("ny", (tau - 0.5) / 3), ("rho_fyz", 1.0), ("Cm", 1.0), ("Cl", 1.0),
                      ("Cu_inverse", 1.0), ("iterations", steps), ("err_every_it", 20), ("plot_every_it", -1),
-                     ("probe_every_it", -1), ("collision", abi.COLLISION_CUM2015), ("closure", abi.CLOSURE_NONE),
+                     ("iterationsMomentAvg", 10), ("probe_iterations", 30), ("ProbeLocation_x", 5),
+                     ("ProbeLocation_y", 7), ("ProbeLocation_z", 4),
+                     ("probe_every_it", 5), ("collision", abi.COLLISION_CUM2015), ("closure", abi.CLOSURE_NONE),
                      ("storage", abi.STORAGE_F64), ("arith", abi.ARITH_LITERAL), ("compat", abi.COMPAT_REFERENCE)):
             assert L.lbmh_case_set(h, k.encode(), float(v)) == 0, k
         assert L.lbmh_case_set_results_dir(h, str(tmp_path).encode()) == 0
@@ -48,6 +50,14 @@ def test_case_cavity_run_matches_oracle(gpu, tmp_path):
         errs = (ctypes.c_double * 8)()
         ne = ctypes.c_int32(8)
         assert L.lbmh_case_errors(h, errs, ctypes.byref(ne)) == 0
+        units = {}
+        for k in ("Cu", "Crho", "Cpressure", "cs2"):
+            v = ctypes.c_double()
+            assert L.lbmh_case_get(h, k.encode(), ctypes.byref(v)) == 0
+            units[k] = v.value
+        avg = [np.empty((n + 2,) * 3) for _ in range(4)]
+        cnt = ctypes.c_int64()
+        assert L.lbmh_case_time_avg(h, *[abi.ptr(a) for a in avg], ctypes.byref(cnt)) == 0
     finally:
         L.lbmh_case_destroy(h)
     oflags, bc, ux = cavity_setup(n)
@@ -63,10 +73,38 @@ def test_case_cavity_run_matches_oracle(gpu, tmp_path):
     ref_err.append(o.error_quadratic())
     o.step(20)
     ref_err.append(o.error_quadratic())
-    o.step(steps - 41)
+    # momentAdd after the steps k > iterations - iterationsMomentAvg = 50
+    o.step(10)
+    o.step_moment_add(steps - 51)
     mo = o.macroscopic()
     for a, b in zip(out, mo):
         assert np.array_equal(a, b)
+    # probe rows at k = 30, 35, ..., 55 (k % probe_every_it == 0, k >= iterations -
+    # probe_iterations), written from device records after the run
+    rows = (tmp_path / "probeData.csv").read_text().splitlines()
+    assert rows[1] == "ux,uy,uz,rho,pressure"
+    vals = np.array([[float(t) for t in r.split(",")] for r in rows[2:]])
+    op = OracleLattice(make_desc(oflags, bc, collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
+                                 arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=float(np.float32(tau))))
+    op.init_equilibrium(None, ux, None, None)
+    want = []
+    done = 0
+    for k in range(30, 60, 5):
+        op.step(k + 1 - done)
+        done = k + 1
+        m = op.macroscopic()
+        r, vx, vy, vz = (float(np.float32(m[j][4, 7, 5])) for j in range(4))
+        f32 = np.float32
+        want.append([f32(f32(vx) * f32(units["Cu"])), f32(f32(vy) * f32(units["Cu"])), f32(f32(vz) * f32(units["Cu"])),
+                     f32(f32(r) * f32(units["Crho"])), f32(f32(r) * f32(units["cs2"]) * f32(units["Cpressure"]))])
+    print(f"[host case] probe rows {len(vals)}", flush=True)
+    assert vals.shape == (6, 5)
+    assert np.allclose(vals, np.array(want, dtype=np.float64), rtol=2e-5, atol=1e-9)
+    oavg, ocnt = o.moment_avg()
+    assert cnt.value == ocnt == 9
+    fluid = oflags != 0
+    for a, b in zip(avg, oavg):
+        assert np.array_equal(a[fluid], b[fluid])
     got = [errs[i] for i in range(ne.value)]
     print(f"[host case] errors {got} oracle {ref_err}", flush=True)
     assert len(got) >= 2
