@@ -94,7 +94,10 @@ enum {
     LBM_BC_INLET_MOVING_WALL = 1,    /* InletVelocityMovingWall.h:31-53              */
     LBM_BC_INLET_EQUILIBRIUM = 2,    /* InletVelocityEquilibrium.h:51-103            */
     LBM_BC_OUTLET_INTERPOLATED = 3,  /* OutletDensityInterpolated.h:170-252          */
-    LBM_BC_PERIODIC_GHOST = 4        /* Periodic.h:43-138                            */
+    LBM_BC_PERIODIC_GHOST = 4,       /* Periodic.h:43-138                            */
+    LBM_BC_OUTLET_EQUILIBRIUM = 5,   /* OutletDensityEquilibrium.h:55-76             */
+    LBM_BC_OUTLET_NEIGHBOUR_EQ = 6,  /* OutletNeighbourEquilibrium.h:78-126          */
+    LBM_BC_OUTLET_NEIGHBOUR_EQ_RF = 7 /* OutletNeighbourEquilibriumOmegaRF.h:143-198 */
 };
 
 /* One boundary-list entry; mirrors the reference PODs boundaryConditionWall /
@@ -125,6 +128,12 @@ typedef struct lbm_desc {
     double rho0;               /* reference density of the deviation storage */
     const int32_t* flags;      /* host, reference encoding, x fastest, nx*ny*nz; copied */
     const lbm_bc_node* bc;     /* host, copied; may be NULL when n_bc == 0 */
+    /* Outlet omega damping (OutletNeighbourEquilibriumOmega.h:18-79, used by the
+     * ...Omega and ...OmegaRF outlets): omega -= (high - low) / m * p over the last
+     * m = ceil(0.1 n) cells of the axis of the first outlet entry's normal, p = 0..m-1,
+     * at the high end of that axis.  Needs an axis-aligned normal. */
+    int32_t outlet_sponge;     /* 1 = on */
+    float sponge_low, sponge_high;   /* LBMConstants::omegaDumpingLow / High */
 } lbm_desc;
 
 /* Create a context on the current HIP device.  Validates the flag grid and the

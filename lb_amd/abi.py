@@ -27,6 +27,7 @@ ARITH_LITERAL, ARITH_FAST = 0, 1
 COMPAT_CREV20, COMPAT_F32CONST, COMPAT_LIM210, COMPAT_TRACE17 = 0x1, 0x2, 0x4, 0x8
 COMPAT_REFERENCE = COMPAT_CREV20 | COMPAT_F32CONST | COMPAT_LIM210 | COMPAT_TRACE17
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
+BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF = 5, 6, 7
 
 STORAGE_BYTES = {STORAGE_F64: 8, STORAGE_F32: 4, STORAGE_F16DEV: 2, STORAGE_BF16DEV: 2, STORAGE_F16DEV_SR: 2,
                  STORAGE_BF16DEV_SR: 2}
@@ -48,7 +49,8 @@ class Desc(ctypes.Structure):
                 ("arith", ctypes.c_int32), ("compat", ctypes.c_uint32), ("n_bc", ctypes.c_int32),
                 ("tau", ctypes.c_double), ("nu", ctypes.c_double), ("c_les", ctypes.c_double),
                 ("rho0", ctypes.c_double),
-                ("flags", ctypes.POINTER(ctypes.c_int32)), ("bc", ctypes.POINTER(BcNode))]
+                ("flags", ctypes.POINTER(ctypes.c_int32)), ("bc", ctypes.POINTER(BcNode)),
+                ("outlet_sponge", ctypes.c_int32), ("sponge_low", ctypes.c_float), ("sponge_high", ctypes.c_float)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

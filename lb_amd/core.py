@@ -23,7 +23,7 @@ def _check(status: int, where: str):
 
 def make_desc(flags: np.ndarray, bc=None, *, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_NONE,
               storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE,
-              periodic=(0, 0, 0), tau=0.6, nu=0.0, c_les=0.0, rho0=1.0):
+              periodic=(0, 0, 0), tau=0.6, nu=0.0, c_les=0.0, rho0=1.0, sponge=None):
     """Build an lbm_desc.  flags: int32 array shaped (nz, ny, nx) (x fastest in memory)."""
     flags = np.ascontiguousarray(flags, dtype=np.int32)
     nz, ny, nx = flags.shape
@@ -32,6 +32,8 @@ def make_desc(flags: np.ndarray, bc=None, *, collision=abi.COLLISION_CUM2015, cl
     d.periodic[:] = [int(bool(p)) for p in periodic]
     d.collision, d.closure, d.storage, d.arith, d.compat = collision, closure, storage, arith, compat
     d.tau, d.nu, d.c_les, d.rho0 = float(tau), float(nu), float(c_les), float(rho0)
+    if sponge is not None:   # (omegaDumpingLow, omegaDumpingHigh) of the outlet omega damping
+        d.outlet_sponge, d.sponge_low, d.sponge_high = 1, float(sponge[0]), float(sponge[1])
     d.flags = flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     bc = bc if bc is not None else []
     if len(bc):

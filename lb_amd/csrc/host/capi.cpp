@@ -200,6 +200,7 @@ Field field(LBMConstants& C, const std::string& n)
     F(BBminx) F(BBmaxx) F(BBminy) F(BBmaxy) F(BBminz) F(BBmaxz) F(resolution_factor) F(additional_factor)
     F(ny_fyz) F(u_guess_fyz) F(L_fyz) F(rho_fyz) F(Re) F(conversion_factor_fyz) F(tau) F(ny) F(omega) F(U_lb)
     F(Cl) F(Ct) F(Cm) F(Cu_inverse) F(Crho) F(Cpressure) F(Cu) F(CLES) F(err) F(time) F(plot_every) F(err_every)
+    F(omegaDumpingLow) F(omegaDumpingHigh) I(outlet_sponge)
     I(dimX_int) I(dimY_int) I(dimZ_int) I(inlet_num) I(outlet_num) I(periodic_num) I(wall_num) I(plot_every_it)
     I(err_every_it) I(iterations) I(iterationsMomentAvg) I(probe_every_it) I(probe_iterations) I(Nvel)
     I(collision) I(closure) I(storage) I(arith)

@@ -83,6 +83,8 @@ struct LBMConstants {
     RealType Cl = 1, Ct = 1, Cm = 1, Cu_inverse = 1, Crho = 1, Cpressure = 1, Cu = 1;
     Vector VelocityInit;
     RealType CLES = 0.173f;  // the reference's const member (:70); settable here
+    RealType omegaDumpingLow = -1.f, omegaDumpingHigh = -1.f;  // LBMConstants.h:74-76
+    int outlet_sponge = 0;   // 1: the ...Omega outlets' omega damping (outletOmega)
     // simulation
     RealType err = 0;
     RealType time = 0, plot_every = -1.f, err_every = -1.f;

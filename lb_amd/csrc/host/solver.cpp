@@ -121,6 +121,9 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
     d.tau = Constants.tau;
     d.nu = Constants.ny;
     d.c_les = Constants.CLES;
+    d.outlet_sponge = Constants.outlet_sponge;
+    d.sponge_low = (float)Constants.omegaDumpingLow;
+    d.sponge_high = (float)Constants.omegaDumpingHigh;
     d.rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
     d.flags = Data.meshFluidHost.data();
     d.bc = bc.empty() ? nullptr : bc.data();

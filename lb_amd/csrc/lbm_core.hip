@@ -77,6 +77,8 @@ struct lbm_ctx {
     double* probe_buf = nullptr;  // [PROBE_CAP][4] on the device
     int probe_n = 0;              // records in probe_buf
     std::vector<double> probe_host;  // records drained from the device, oldest first
+    // outlet omega damping
+    int sp_axis = -1, sp_begin = 0, sp_interval = 1;
 };
 
 static constexpr int PROBE_CAP = 4096;
@@ -143,6 +145,9 @@ static AuxArgs<ST> aux_args(const lbm_ctx* c)
     a.odd = (int)(c->steps & 1);
     a.first = c->first ? 1 : 0;
     a.io = c->stage;
+    a.sp_axis = c->sp_axis;
+    a.sp_begin = c->sp_begin;
+    a.sp_deriv = ((R)c->d.sponge_high - (R)c->d.sponge_low) / (R)c->sp_interval;
     return a;
 }
 
@@ -237,6 +242,9 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.partials = c->partials;
     a.ccache = c->ccache;
     a.g = c->g;
+    a.sp_axis = c->sp_axis;
+    a.sp_begin = c->sp_begin;
+    a.sp_deriv = ((R)c->d.sponge_high - (R)c->d.sponge_low) / (R)c->sp_interval;
     a.omega0 = R(1) / (R)c->d.tau;   // 1.f / tau (OmegaNo.h:44, NonDim...h:114)
     a.tau = (R)c->d.tau;
     a.cles = (R)c->d.c_les;
@@ -505,6 +513,33 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
             P.outl.push_back(e);
             break;
         }
+        case LBM_BC_OUTLET_EQUILIBRIUM:
+        case LBM_BC_OUTLET_NEIGHBOUR_EQ:
+        case LBM_BC_OUTLET_NEIGHBOUR_EQ_RF: {
+            if (kind[i] == K_SOLID) return LBM_E_ARG;
+            if (kind[i] == K_INLET || has_outlet[i]) return LBM_E_UNSUPPORTED;
+            has_outlet[i] = 1;
+            NodeDev e = node_dev(L, b);
+            e.s0 = pub_slot(i);   // the node's own rho, u
+            uint32_t mask = 0;
+            if (b.kind != LBM_BC_OUTLET_EQUILIBRIUM) {
+                for (int q = 0; q < Q; ++q) {
+                    const float nc = b.nx * (float)CX[q] + b.ny * (float)CY[q] + b.nz * (float)CZ[q];
+                    if (!(nc < 0.0f)) continue;
+                    int x1, y1, z1;
+                    // the reference reads outside its arrays here (no bounds checks); skipped
+                    if (!L.nbr(b.x, b.y, b.z, CX[q], CY[q], CZ[q], x1, y1, z1)) continue;
+                    const size_t j = L.user(x1, y1, z1);
+                    if (!b.regular && !(d->flags[j] > 0)) continue;
+                    if (kind[j] == K_INLET) return LBM_E_UNSUPPORTED;
+                    if (kind[j] != K_SOLID) pub_slot(j);
+                    mask |= 1u << q;
+                }
+            }
+            e.partner = (int32_t)mask;
+            P.outl.push_back(e);
+            break;
+        }
         case LBM_BC_PERIODIC_GHOST: {
             if (kind[i] == K_SOLID) return LBM_E_ARG;
             if (kind[i] == K_INLET) return LBM_E_UNSUPPORTED;
@@ -525,6 +560,17 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
         }
         default:
             return LBM_E_ARG;
+        }
+    }
+    // a neighbour-equilibrium entry must not read a cell that another outlet
+    // entry of the same launch writes
+    for (const NodeDev& e : P.outl) {
+        if (e.kind != LBM_BC_OUTLET_NEIGHBOUR_EQ && e.kind != LBM_BC_OUTLET_NEIGHBOUR_EQ_RF) continue;
+        for (int q = 0; q < Q; ++q) {
+            if (!(((uint32_t)e.partner >> q) & 1u)) continue;
+            int x1, y1, z1;
+            L.nbr(e.x - L.px, e.y - L.py, e.z - L.pz, CX[q], CY[q], CZ[q], x1, y1, z1);
+            if (has_outlet[L.user(x1, y1, z1)]) return LBM_E_UNSUPPORTED;
         }
     }
     // flag -1 cells without an inlet entry keep their state and stream it
@@ -718,6 +764,7 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     if (d->closure == LBM_CLOSURE_MLP && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->closure == LBM_CLOSURE_MLP && d->storage == LBM_STORAGE_F64) return LBM_E_UNSUPPORTED;
     if (d->closure == LBM_CLOSURE_MLP && d->collision > LBM_COLLISION_CUM2015) return LBM_E_UNSUPPORTED;
+    if (d->closure == LBM_CLOSURE_MLP && d->outlet_sponge) return LBM_E_UNSUPPORTED;
     if (d->storage < LBM_STORAGE_F64 || d->storage > LBM_STORAGE_BF16DEV_SR) return LBM_E_ARG;
     if (d->arith != LBM_ARITH_LITERAL && d->arith != LBM_ARITH_FAST) return LBM_E_ARG;
     if (d->arith == LBM_ARITH_LITERAL && d->storage >= LBM_STORAGE_F16DEV) return LBM_E_ARG;
@@ -773,6 +820,31 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     c->n_inlet = (int)P.inl.size();
     c->n_eq = P.n_eq;
     c->n_outlet = (int)P.outl.size();
+    if (d->outlet_sponge) {
+        // OutletNeighbourEquilibriumOmega::outletOmega (:18-79): the first outlet
+        // entry's normal picks the axis; the last ceil(0.1 n) cells of it
+        const lbm_bc_node* b0 = nullptr;
+        for (int n = 0; n < d->n_bc && !b0; ++n)
+            if (d->bc[n].kind == LBM_BC_OUTLET_INTERPOLATED || d->bc[n].kind == LBM_BC_OUTLET_EQUILIBRIUM ||
+                d->bc[n].kind == LBM_BC_OUTLET_NEIGHBOUR_EQ || d->bc[n].kind == LBM_BC_OUTLET_NEIGHBOUR_EQ_RF)
+                b0 = &d->bc[n];
+        if (!b0) {
+            lbm_destroy(c);
+            return LBM_E_ARG;
+        }
+        const float an[3] = {std::fabs(b0->nx), std::fabs(b0->ny), std::fabs(b0->nz)};
+        int axis = -1;
+        for (int k = 0; k < 3; ++k)
+            if (an[k] == 1.0f && an[(k + 1) % 3] == 0.0f && an[(k + 2) % 3] == 0.0f) axis = k;
+        if (axis < 0) {
+            lbm_destroy(c);
+            return LBM_E_UNSUPPORTED;
+        }
+        const int dim = axis == 0 ? d->nx : axis == 1 ? d->ny : d->nz;
+        c->sp_interval = (int)std::ceil(dim * 0.1f);
+        c->sp_begin = dim - c->sp_interval;
+        c->sp_axis = axis;
+    }
     c->n_ghost = (int)P.ghost.size();
     c->n_pub = P.n_pub;
     c->ghost_off = P.ghost_off;
@@ -811,7 +883,7 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->inl, P.inl.size() * sizeof(NodeDev));
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->outl, P.outl.size() * sizeof(NodeDev));
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->ghost, P.ghost.size() * sizeof(NodeDev));
-    if (st == LBM_OK) st = dev_alloc(c, &c->pub_mac, (size_t)c->n_pub * 2 * rb);
+    if (st == LBM_OK) st = dev_alloc(c, &c->pub_mac, (size_t)c->n_pub * PUB_MAC * rb);
     if (st == LBM_OK) st = dev_alloc(c, &c->pub_post, (size_t)c->n_pub * Q * rb);
     if (st == LBM_OK) st = dev_alloc(c, &c->macro, 4 * N * rb);
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->stage, 5 * NU * sizeof(double));
@@ -833,7 +905,7 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     ok = ok && hipMemsetAsync(c->pops, 0, N * elem * Q, c->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(c->inl_state, 0, std::max<size_t>(16, (size_t)c->n_inlet * Q * rb), c->stream) ==
                    hipSuccess;
-    ok = ok && hipMemsetAsync(c->pub_mac, 0, std::max<size_t>(16, (size_t)c->n_pub * 2 * rb), c->stream) ==
+    ok = ok && hipMemsetAsync(c->pub_mac, 0, std::max<size_t>(16, (size_t)c->n_pub * PUB_MAC * rb), c->stream) ==
                    hipSuccess;
     ok = ok && hipMemsetAsync(c->pub_post, 0, std::max<size_t>(16, (size_t)c->n_pub * Q * rb), c->stream) ==
                    hipSuccess;

@@ -61,6 +61,7 @@ constexpr int TPB = 256;
 // bit 62 the Combined operator's 2017 branch (flag != 1), bit 63 wall-list node
 constexpr uint64_t MASK27 = (1ull << 27) - 1;
 constexpr int NW_CUM2017 = 62;
+constexpr int PUB_MAC = 4;   // published rho, ux, uy, uz per publish slot
 
 struct Geo {
     int nx, ny, nz;     // padded lattice
@@ -96,7 +97,7 @@ struct StepArgs {
     const int32_t* bcidx;          // inlet: state index; CL_PUB: publish slot
     typename ST::R* inl_state;     // [n_inlet][27]
     const NodeDev* inl;            // [n_inlet]
-    typename ST::R* pub_mac;       // [n_pub][2]: rho, u_x
+    typename ST::R* pub_mac;       // [n_pub][PUB_MAC]: rho, u_x, u_y, u_z
     typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
     const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
     double* partials;              // [ntiles][8] when diag != 0: mass, momentum, wall force
@@ -112,7 +113,21 @@ struct StepArgs {
     int diag;
     int closure;
     uint32_t seed;                 // per-step seed of the stochastic-rounding stores
+    int sp_axis, sp_begin;         // outlet omega damping: axis (-1 off) and first cell
+    typename ST::R sp_deriv;       // omega decrement per cell past sp_begin
 };
+
+// Outlet omega damping (OutletNeighbourEquilibriumOmega.h:18-79) of a cell at
+// padded coordinates: omega - deriv * (coordinate - begin) past begin.
+template <class R>
+__device__ __forceinline__ R sponge(R omega, int axis, int begin, R deriv, const Geo& g, int x, int y, int z)
+{
+    LBM_FP_STRICT
+    if (axis < 0) return omega;
+    const int v = axis == 0 ? x - g.px : axis == 1 ? y - g.py : z - g.pz;
+    if (v < begin) return omega;
+    return omega - deriv * (R)(v - begin);
+}
 
 // Arguments of the collide-stream step with the learned closure.
 template <class ST>
@@ -296,7 +311,8 @@ __device__ __forceinline__ void moment_add(double* avg, size_t nu, size_t iu, do
 template <class ST, bool ODD, bool FLAGS>
 __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c, uint8_t cls, int bi, bool inlet,
                                             const typename ST::R (&f)[Q], const typename ST::R (&P)[Q],
-                                            typename ST::R rho, typename ST::R ux)
+                                            typename ST::R rho, typename ST::R ux, typename ST::R uy,
+                                            typename ST::R uz)
 {
     using R = typename ST::R;
     const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
@@ -342,8 +358,10 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
             }
             // data the boundary patch kernels of this step read
             if (cls == CL_PUB) {
-                a.pub_mac[(size_t)bi * 2] = rho;
-                a.pub_mac[(size_t)bi * 2 + 1] = ux;
+                a.pub_mac[(size_t)bi * PUB_MAC] = rho;
+                a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
+                a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
+                a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
                 R* dst = a.pub_post + (size_t)bi * Q;
 #pragma unroll
                 for (int q = 0; q < Q; ++q) dst[q] = P[q];
@@ -398,6 +416,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                            (double)uy, (double)uz);
             omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
                                                          : a.omega0;
+            omega = sponge(omega, a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
 #pragma unroll
             for (int q = 0; q < Q; ++q) P[q] = f[q];
             if (COLL == LBM_COLLISION_SRT)
@@ -441,6 +460,7 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 omega = omega_from_coefficient((R)a.ccache[c.i], pin_over_rho(ce, rho), a.tau);
             else
                 omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+            omega = sponge(omega, a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
             if (COLL == LBM_COLLISION_SRT) {
 #pragma unroll
                 for (int q = 0; q < Q; ++q) P[q] = f[q];
@@ -512,8 +532,10 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 }
                 // data the boundary patch kernels of this step read
                 if (cls == CL_PUB) {
-                    a.pub_mac[(size_t)bi * 2] = rho;
-                    a.pub_mac[(size_t)bi * 2 + 1] = ux;
+                    a.pub_mac[(size_t)bi * PUB_MAC] = rho;
+                    a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
+                    a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
+                    a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
                     R* dst = a.pub_post + (size_t)bi * Q;
 #pragma unroll
                     for (int q = 0; q < Q; ++q) dst[q] = P[q];
@@ -640,7 +662,7 @@ __global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
             for (int q = 0; q < Q; ++q) P[q] = f[q];
             srt_fast(a.K, P, cp.rho, cp.ux, cp.uy, cp.uz, omega);
         }
-        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, f, P, cp.rho, cp.ux);
+        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, f, P, cp.rho, cp.ux, cp.uy, cp.uz);
     }
 }
 
@@ -709,9 +731,9 @@ __global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
     const NodeDev b = a.nodes[j];
     if (b.kind != LBM_BC_INLET_EQUILIBRIUM) return;
     const R uxb = b.ux, uyb = b.uy, uzb = b.uz;
-    const R rho1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * 2] : R(0);
-    const R ux1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * 2 + 1] : R(0);
-    const R ux2 = b.s1 >= 0 ? a.pub_mac[(size_t)b.s1 * 2 + 1] : R(0);
+    const R rho1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * PUB_MAC] : R(0);
+    const R ux1 = b.s0 >= 0 ? a.pub_mac[(size_t)b.s0 * PUB_MAC + 1] : R(0);
+    const R ux2 = b.s1 >= 0 ? a.pub_mac[(size_t)b.s1 * PUB_MAC + 1] : R(0);
     R rhob;
     if (b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f) {
         const R duxb = (uxb - ux1);
@@ -733,6 +755,59 @@ __global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
 // (x, y - c_y, z - c_z) is solid (sequentially in q), then incoming
 // populations are interpolated between the node's and the upstream cell's
 // post-collision values and the density is reset to the outlet density.
+// OutletDensityEquilibrium (OutletDensityEquilibrium.h:55-76) and the
+// neighbour-equilibrium outlets (OutletNeighbourEquilibrium.h:78-126,
+// ...OmegaRF.h:143-198) on one list entry.  rho, u of a cell are the moments
+// its collision used in this step (published by the step kernel), which is
+// what the reference's rho/u fields hold at this point.  b.partner holds the
+// mask of incoming directions the rule acts on (host-computed: n.c < 0, the
+// neighbour x + c inside the lattice and, at irregular nodes, flagged > 0).
+template <class ST>
+__device__ __forceinline__ void outlet_equilibrium(const PatchArgs<ST>& a, const NodeDev& b)
+{
+    LBM_FP_STRICT
+    using R = typename ST::R;
+    const Cell c = cell_at(a.g, b.x, b.y, b.z);
+    R d[Q];
+    int pl[Q];
+    uint32_t ix[Q];
+    load_node(a, c, d, pl, ix);
+    const R* mx = a.pub_mac + (size_t)b.s0 * PUB_MAC;
+    const R ux = mx[1], uy = mx[2], uz = mx[3];
+    R density = b.rho;
+    if (b.kind == LBM_BC_OUTLET_EQUILIBRIUM) {
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel) {
+            const float nc = b.nx * (float)CX[vel] + b.ny * (float)CY[vel] + b.nz * (float)CZ[vel];
+            if (nc > 0.0f) d[crev_of(vel, a.crev20 != 0)] = feq_q(a.K, vel, ux, uy, uz, density);
+        }
+    } else {
+        // the RF variant's backflow correction is written with a double literal 0.5
+        if (b.kind == LBM_BC_OUTLET_NEIGHBOUR_EQ_RF && ux < R(0))
+            density = (R)((double)density - 0.5 * (double)ux * (double)ux * (double)density);
+        const uint32_t mask = (uint32_t)b.partner;
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel) {
+            if (!((mask >> vel) & 1u)) continue;
+            const int xn = wrap(b.x + CX[vel], a.g.nx, a.g.wx), yn = wrap(b.y + CY[vel], a.g.ny, a.g.wy),
+                      zn = wrap(b.z + CZ[vel], a.g.nz, a.g.wz);
+            const Cell nc = cell_at(a.g, xn, yn, zn);
+            R fEqNeigh = R(0), fNeigh = R(0);   // a solid neighbour: rho = 0, f = 0 in the reference
+            if (a.cls[nc.i] != CL_SOLID) {
+                const R* mn = a.pub_mac + (size_t)a.bcidx[nc.i] * PUB_MAC;
+                fEqNeigh = feq_q(a.K, vel, mn[1], mn[2], mn[3], mn[0]);
+                int pn;
+                uint32_t in;
+                read_slot(nc, vel, a.odd != 0, pn, in);
+                fNeigh = ST::load(a.p[pn], in, a.rest[WCLASS[vel]]);
+            }
+            const R fEqBC = feq_q(a.K, vel, ux, uy, uz, density);
+            d[vel] = fEqBC + (fEqNeigh - fNeigh);
+        }
+    }
+    store_node(a, d, pl, ix);
+}
+
 template <class ST>
 __global__ __launch_bounds__(TPB) void k_outlet(PatchArgs<ST> a)
 {
@@ -741,6 +816,10 @@ __global__ __launch_bounds__(TPB) void k_outlet(PatchArgs<ST> a)
     const int j = blockIdx.x * TPB + threadIdx.x;
     if (j >= a.n) return;
     const NodeDev b = a.nodes[j];
+    if (b.kind != LBM_BC_OUTLET_INTERPOLATED) {
+        outlet_equilibrium(a, b);
+        return;
+    }
     const bool xp = b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f;
     const bool xm = b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f;
     if (!xp && !xm) return;
@@ -859,6 +938,8 @@ struct AuxArgs {
     int first;      // no step taken since lbm_init_equilibrium: rho, u are the macro planes
     int q;          // plane for gather/scatter
     double* io;     // staging buffer (FP64, caller layout)
+    int sp_axis, sp_begin;      // outlet omega damping (as StepArgs)
+    typename ST::R sp_deriv;
 };
 
 // Thread -> caller cell -> padded cell.
@@ -977,8 +1058,9 @@ __device__ __forceinline__ void cell_macro(const AuxArgs<ST>& a, const Cell& c, 
             moments_literal(f, rho, ux, uy, uz);
         }
         if (om)
-            *om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
-                                                       : a.omega0;
+            *om = sponge(a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
+                                                              : a.omega0,
+                         a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
     } else {
         const Raw2<R> r = raw_moments(f);
         if (a.first) {
@@ -1002,7 +1084,8 @@ __device__ __forceinline__ void cell_macro(const AuxArgs<ST>& a, const Cell& c, 
                 vz = r.jz * ir;
             }
             const Cen2<R> ce = central2(r, vx, vy, vz);
-            *om = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
+            *om = sponge(a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0,
+                         a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
         }
     }
 }

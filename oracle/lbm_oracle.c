@@ -128,6 +128,8 @@ int orc_get_threads(void)
         (S)->per[2] = d->periodic[2] != 0;                                                  \
         (S)->collision = d->collision; (S)->closure = d->closure; (S)->compat = compat;     \
         (S)->tau = (T)d->tau; (S)->nu = (T)d->nu; (S)->CLES = (T)d->c_les;                  \
+        (S)->sponge = d->outlet_sponge != 0;                                                \
+        (S)->sponge_low = (T)d->sponge_low; (S)->sponge_high = (T)d->sponge_high;           \
         ORC_FILL_CONSTS(S, T);                                                              \
         (S)->flags = (int*)malloc(sizeof(int) * N);                                         \
         (S)->df = (T*)calloc((size_t)27 * N, sizeof(T));                                    \
@@ -304,12 +306,15 @@ int orc_get_post_populations(orc* o, double* f)
 int orc_get_omega(orc* o, double* w)
 {
     if (!o || !w) return LBM_E_ARG;
-    /* the rate the next step would use: Turbulence::omega on the current state */
+    /* the rate the next step would use: Turbulence::omega on the current state,
+     * then the outlet's omega damping (SolverTurbulentLES.h:188-193) */
     if (o->dbl) {
         op_omega_d(&o->d);
+        op_outlet_omega_d(&o->d);
         for (long i = 0; i < o->d.N; ++i) w[i] = o->d.flags[i] != 0 ? o->d.omega[i] : 0.0;
     } else {
         op_omega_f(&o->f);
+        op_outlet_omega_f(&o->f);
         for (long i = 0; i < o->f.N; ++i) w[i] = o->f.flags[i] != 0 ? o->f.omega[i] : 0.0;
     }
     return LBM_OK;

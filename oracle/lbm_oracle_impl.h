@@ -24,6 +24,8 @@ typedef struct FN(state) {
     uint32_t compat;
     REAL tau, nu, CLES, c13, cs;
     FN(ck) ck;
+    int sponge;             /* outlet omega damping on */
+    REAL sponge_low, sponge_high;
     REAL w[27];
     int crev[27];
     int* flags;
@@ -523,10 +525,86 @@ static void FN(op_inlet)(FN(state) * s)
 }
 
 /* OutletDensityInterpolated::outlet (OutletDensityInterpolated.h:170-252) */
+/* f_equilibrium_outlet / f_equilibrium of the outlet variants
+ * (OutletNeighbourEquilibrium.h:35-74): u of cell i, the given density. */
+static inline REAL FN(feq_at)(const FN(state) * s, int vel, long i, REAL density)
+{
+    return FN(feq)(s->w, vel, s->u[3 * i], s->u[3 * i + 1], s->u[3 * i + 2], density);
+}
+
+/* OutletDensityEquilibrium::outlet (OutletDensityEquilibrium.h:55-76): for
+ * every outgoing direction (n.c > 0) the slot c_rev[vel] receives the
+ * equilibrium of vel at the target density and the node's velocity.
+ * OutletNeighbourEquilibrium::outlet (OutletNeighbourEquilibrium.h:78-126),
+ * ...Omega (:139-186), ...OmegaRF (:143-198): for every incoming direction
+ * (n.c < 0), f = f_eq(rho_target, u(x)) + (f_eq(rho, u)(x + c) - f(x + c)),
+ * at irregular nodes only where the neighbour's flag is > 0; the RF variant
+ * lowers the target density by rho u_x^2 / 2 where u_x(x) < 0. */
+static void FN(op_outlet_eq)(FN(state) * s, const lbm_bc_node* b)
+{
+    long i = FN(idx)(s, b->x, b->y, b->z);
+    REAL density = b->rho;
+    if (b->kind == LBM_BC_OUTLET_EQUILIBRIUM) {
+        for (int vel = 0; vel < 27; ++vel)
+            if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] > 0)
+                DF(s, s->crev[vel], i) = FN(feq_at)(s, vel, i, density);
+        return;
+    }
+    if (b->kind == LBM_BC_OUTLET_NEIGHBOUR_EQ_RF && s->u[3 * i] < 0)
+        density = density - 0.5 * s->u[3 * i] * s->u[3 * i] * density;
+    for (int vel = 0; vel < 27; ++vel) {
+        if (!(b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] < 0)) continue;
+        long j;
+        /* the reference has no bounds checks; a neighbour outside the array is skipped */
+        if (!FN(nb)(s, b->x + ORC_C[vel][0], b->y + ORC_C[vel][1], b->z + ORC_C[vel][2], &j)) continue;
+        if (!b->regular && !(s->flags[j] > 0)) continue;
+        REAL fEqNeigh = FN(feq_at)(s, vel, j, s->rho[j]);
+        REAL fNeqNeigh = DF(s, vel, j);
+        REAL fEqBC = FN(feq_at)(s, vel, i, density);
+        DF(s, vel, i) = fEqBC + (fEqNeigh - fNeqNeigh);
+    }
+}
+
+/* OutletNeighbourEquilibriumOmega::outletOmega (:18-79): omega decreases
+ * linearly over the last 10 % of the lattice along the first outlet entry's
+ * normal axis (at the high end of that axis whatever the normal's sign):
+ * omega -= (high - low) / interval * position, position = 0 .. interval - 1. */
+static void FN(op_outlet_omega)(FN(state) * s)
+{
+    if (!s->sponge) return;
+    const lbm_bc_node* b0 = 0;
+    for (int n = 0; n < s->n_bc && !b0; ++n)
+        if (s->bc[n].kind >= LBM_BC_OUTLET_INTERPOLATED && s->bc[n].kind != LBM_BC_PERIODIC_GHOST) b0 = &s->bc[n];
+    if (!b0) return;
+    const float nx = b0->nx, ny = b0->ny, nz = b0->nz;
+    const float px = fabsf(fabsf(nx) - 1), py = fabsf(fabsf(ny) - 1), pz = fabsf(fabsf(nz) - 1);
+    const int dx01 = (int)ceilf(s->nx * 0.1f), dy01 = (int)ceilf(s->ny * 0.1f), dz01 = (int)ceilf(s->nz * 0.1f);
+    const int dX = (int)(dx01 * fabsf(nx) + s->nx * fabsf(px));
+    const int dY = (int)(dy01 * fabsf(ny) + s->ny * fabsf(py));
+    const int dZ = (int)(dz01 * fabsf(nz) + s->nz * fabsf(pz));
+    const int bX = s->nx - dX, bY = s->ny - dY, bZ = s->nz - dZ;
+    const int interval = (int)fabsf(dx01 * nx + dy01 * ny + dz01 * nz);
+    const REAL lo = s->sponge_low, hi = s->sponge_high;
+    const REAL derivation = (hi - lo) / interval;
+    for (int z = bZ; z < s->nz; ++z)
+        for (int y = bY; y < s->ny; ++y)
+            for (int x = bX; x < s->nx; ++x) {
+                const int pos = (int)((x - bX) * nx * nx + (y - bY) * ny * ny + (z - bZ) * nz * nz);
+                const REAL damp = derivation * pos;
+                long i = FN(idx)(s, x, y, z);
+                s->omega[i] = s->omega[i] - damp;
+            }
+}
+
 static void FN(op_outlet)(FN(state) * s)
 {
     for (int n = 0; n < s->n_bc; ++n) {
         const lbm_bc_node* b = &s->bc[n];
+        if (b->kind == LBM_BC_OUTLET_EQUILIBRIUM || b->kind == LBM_BC_OUTLET_NEIGHBOUR_EQ ||
+            b->kind == LBM_BC_OUTLET_NEIGHBOUR_EQ_RF) {
+            FN(op_outlet_eq)(s, b);
+            continue;
+        }
         if (b->kind != LBM_BC_OUTLET_INTERPOLATED) continue;
         const REAL density = b->rho;
         long i = FN(idx)(s, b->x, b->y, b->z);
@@ -620,6 +698,7 @@ static void FN(op_moments)(FN(state) * s)
 static void FN(step)(FN(state) * s)
 {
     FN(op_omega)(s);
+    FN(op_outlet_omega)(s);
     FN(op_collision)(s);
     FN(op_streaming)(s);
     FN(op_bounce_back)(s);

@@ -13,6 +13,7 @@ import numpy as np
 
 # lbm_bc_node kinds (include/lbm_core.h)
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
+BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF = 5, 6, 7
 
 
 def meshing_boundary_wall(grid: np.ndarray):
@@ -131,6 +132,7 @@ def cavity(n: int, lid_velocity=(0.05, 0.0, 0.0)):
 
 
 def channel(nx, ny, nz, u_in=0.04, rho_out=1.0, ghost_z=True, obstacle=True, inlet_kind=BC_INLET_EQUILIBRIUM,
+            outlet_kind=BC_OUTLET_INTERPOLATED,
             walls=True):
     """Channel along x: solid rows at y = 0 and y = ny - 1, an equilibrium
     velocity inlet on x = 0 (outward normal -x), an interpolated density
@@ -170,7 +172,7 @@ def channel(nx, ny, nz, u_in=0.04, rho_out=1.0, ghost_z=True, obstacle=True, inl
     walls = compile_wall(flags, walls)
     bc += [dict(kind=BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls]
     bc += [dict(kind=inlet_kind, **d) for d in inl]
-    bc += [dict(kind=BC_OUTLET_INTERPOLATED, **d) for d in out]
+    bc += [dict(kind=outlet_kind, **d) for d in out]
     bc += [dict(kind=BC_PERIODIC_GHOST, **d) for d in per]
     return flags, bc
 

@@ -449,3 +449,71 @@ def test_moment_average_is_the_mean_of_the_snapshots():
     assert n2 == 5
     assert np.allclose(avg2[0], 1.0, atol=1e-14) and np.allclose(avg2[1], 0.03, atol=1e-15)
     assert np.allclose(avg2[2], -0.01, atol=1e-15) and np.allclose(avg2[3], 0.02, atol=1e-15)
+
+
+# --- outlet variants and omega damping ------------------------------------
+
+@pytest.mark.parametrize("kind", [abi.BC_OUTLET_NEIGHBOUR_EQ, abi.BC_OUTLET_NEIGHBOUR_EQ_RF],
+                         ids=["neighbour_eq", "neighbour_eq_rf"])
+def test_uniform_flow_is_a_fixed_point_of_the_neighbour_outlets(kind):
+    """f = f_eq(rho_t, u(x)) + (f_eq(x + c) - f(x + c)) leaves a uniform
+    equilibrium flow at rho_t = 1 unchanged (the RF correction needs u_x < 0)."""
+    u0 = 0.04
+    flags, _, bc = _channel(12, 6, 6, u_in=u0, obstacle=False, walls=False, outlet_kind=kind)
+    o = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    u = np.float64(np.float32(u0))
+    o.init_equilibrium(None, np.full(flags.shape, u), None, None)
+    o.step(20)
+    expect = feq(1.0, (u, 0.0, 0.0))
+    assert np.abs(o.populations() - expect[:, None, None, None]).max() < 1e-15
+
+
+def test_density_equilibrium_outlet_writes_the_outgoing_equilibrium_into_the_incoming_slot():
+    """OutletDensityEquilibrium.h:64-71: slot c_rev[v] of every outgoing v
+    (n.c > 0) holds f_eq(v; rho_t, u(x)) after the step.  At rest the rule is
+    a fixed point; in a flow it is not (f_eq(v) != f_eq(c_rev v))."""
+    c, crev, _ = tables(0)
+    u0 = 0.04
+    flags, _, bc = _channel(12, 6, 6, u_in=u0, obstacle=False, walls=False, outlet_kind=abi.BC_OUTLET_EQUILIBRIUM)
+    o = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    u = np.float64(np.float32(u0))
+    o.init_equilibrium(None, np.full(flags.shape, u), None, None)
+    o.step(1)
+    f = o.populations()
+    eq = feq(1.0, (u, 0.0, 0.0))
+    out_x = flags.shape[2] - 1
+    outgoing = [v for v in range(27) if c[v][0] > 0]
+    for v in outgoing:
+        assert np.allclose(f[crev[v], :, :, out_x], eq[v], atol=1e-15, rtol=0)
+        assert abs(eq[v] - eq[crev[v]]) > 1e-4
+    flags2, _, bc2 = _channel(12, 6, 6, u_in=0.0, obstacle=False, walls=False,
+                              outlet_kind=abi.BC_OUTLET_EQUILIBRIUM)
+    o2 = OracleLattice(make_desc(flags2, bc2, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                 storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    o2.init_equilibrium(None, 0.0, None, None)
+    o2.step(5)
+    assert np.abs(o2.populations() - feq(1.0, (0, 0, 0))[:, None, None, None]).max() < 1e-15
+
+
+def test_outlet_omega_damping_is_a_ramp_over_the_last_tenth():
+    """outletOmega (OutletNeighbourEquilibriumOmega.h:18-79): omega = 1/tau
+    outside the last ceil(0.1 n) cells of the outlet normal's axis and decreases
+    linearly inside them, by (high - low) / ceil(0.1 n) per cell, from 0 at the
+    first damped cell."""
+    nx, tau, lo, hi = 25, 0.6, 0.0, float(np.float32(0.05))   # float constants, as the reference's
+    flags, _, bc = _channel(nx, 6, 6, obstacle=False, walls=False, outlet_kind=abi.BC_OUTLET_NEIGHBOUR_EQ)
+    o = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=tau, nu=0.1 / 3,
+                                sponge=(lo, hi)))
+    o.init_equilibrium(None, 0.02, None, None)
+    o.step(1)
+    w = o.omega()
+    m = int(np.ceil(nx * np.float32(0.1)))   # 3 cells
+    x = np.arange(nx)
+    ramp = np.where(x >= nx - m, (hi - lo) / m * (x - (nx - m)), 0.0)
+    fluid = flags != 0
+    expect = np.broadcast_to(1 / tau - ramp, flags.shape)
+    assert m == 3
+    assert np.allclose(w[fluid], expect[fluid], rtol=0, atol=1e-15)
