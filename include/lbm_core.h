@@ -250,6 +250,30 @@ lbm_status lbm_device_bytes(lbm_ctx* ctx, size_t* bytes);
  * one (name may be NULL).  Returns LBM_E_HIP when no device is visible. */
 lbm_status lbm_device_info(int32_t* count, char* name, size_t name_len);
 
+/* One node of a bounding-volume hierarchy over a triangle mesh: an inner node
+ * has children left/right (count = 0); a leaf holds triangles
+ * order[first .. first + count) (left = right = -1). */
+typedef struct lbm_bvh_node {
+    double lo[3], hi[3];
+    int32_t left, right, first, count;
+} lbm_bvh_node;
+
+/* The OFF voxelizer's inside test on the current device: for every cell
+ * (i, j, k) of a dims[0] x dims[1] x dims[2] grid (x fastest) the query point
+ * ((i + 0.5 - additional) / resolution + bbmin), held in a float as the
+ * reference's RealType is (geometryMesherOFF.h:90-103), is inside when the
+ * segment to `outside` meets the triangles in an odd number of distinct
+ * points, points closer than 1e-5 counted once (geometryHandlerOFF.h:135-182).
+ * grid[cell] = 1 inside, 0 outside.  nodes/order are a hierarchy over the
+ * triangles (the host voxelizer's), traversed in the same order so that hits
+ * are merged in the same order.  Synchronises.  LBM_E_UNSUPPORTED if a
+ * segment needs more than 64 hits or a hierarchy deeper than 64 levels
+ * (none computed). (No reference counterpart as a GPU routine.) */
+lbm_status lbm_voxelize(const double* vertices, int32_t n_vertices, const int32_t* triangles, int32_t n_triangles,
+                        const lbm_bvh_node* nodes, int32_t n_nodes, const int32_t* order, const int32_t dims[3],
+                        double additional, double resolution, const double bbmin[3], const double outside[3],
+                        int32_t* grid);
+
 const char* lbm_status_str(lbm_status s);
 void lbm_destroy(lbm_ctx* ctx);
 

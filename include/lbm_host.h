@@ -30,8 +30,11 @@ extern "C" {
  * in float, query point of cell (i,j,k) = ((i+0.5-add)/res + BBmin) held in
  * float, fluid (1) when the segment to `outside` crosses the surface an odd
  * number of times.  grid may be NULL (sizes only); otherwise it receives
- * dims[0]*dims[1]*dims[2] ints, x fastest.  threads <= 0: all cores.
- * Returns 0, 1 (empty polyhedron) or 2 (unreadable / malformed file). */
+ * dims[0]*dims[1]*dims[2] ints, x fastest.  threads > 0: that many CPU
+ * threads; 0: all cores; < 0: the inside tests run on the current GPU
+ * (lbm_voxelize, same hierarchy and arithmetic, same grid).
+ * Returns 0, 1 (empty polyhedron), 2 (unreadable / malformed file) or, for a
+ * failed GPU run, 10 + its lbm_status. */
 int lbmh_voxelize_off(const char* path, float resolution, float additional, const double outside[3], int threads,
                       int32_t dims[3], float bbmin[3], float bbmax[3], int32_t* grid);
 

@@ -80,6 +80,9 @@ CORE_SYMBOLS = [
     ("lbm_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     ("lbm_bandwidth_probe", ctypes.c_int, [ctypes.c_size_t, ctypes.c_int32, _dp, _dp]),
     ("lbm_device_info", ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]),
+    ("lbm_voxelize", ctypes.c_int, [_dp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, _vp,
+                                    ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.c_double, ctypes.c_double, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]),
     ("lbm_status_str", ctypes.c_char_p, [ctypes.c_int]),
     ("lbm_destroy", None, [_vp]),
 ]

@@ -73,6 +73,7 @@ int lbmh_voxelize_off(const char* path, float resolution, float additional, cons
     if (bbmax) { bbmax[0] = C.BBmaxx; bbmax[1] = C.BBmaxy; bbmax[2] = C.BBmaxz; }
     if (!grid) return 0;
     M.meshingPolyhedron(H, {outside[0], outside[1], outside[2]}, false, threads);
+    if (threads < 0 && M.device_status != LBM_OK) return 10 + M.device_status;
     std::memcpy(grid, D.meshFluidHost.data(), sizeof(int32_t) * D.meshFluidHost.size());
     return 0;
 }

@@ -291,6 +291,33 @@ bool geometryHandlerOFF::polyhedronInside(d3 pnt_ask, d3 pnt_ref) const
     return impl->count_crossings(p, q) % 2 == 1;
 }
 
+int geometryHandlerOFF::insideGridDevice(const int32_t dims[3], double additional, double resolution,
+                                         const double bbmin[3], d3 pnt_ref, int32_t* grid) const
+{
+    const Impl& I = *impl;
+    std::vector<double> V(3 * I.V.size());
+    for (size_t k = 0; k < I.V.size(); ++k)
+        for (int a = 0; a < 3; ++a) V[3 * k + a] = I.V[k][a];
+    std::vector<int32_t> T(3 * I.T.size());
+    for (size_t k = 0; k < I.T.size(); ++k)
+        for (int a = 0; a < 3; ++a) T[3 * k + a] = I.T[k][a];
+    std::vector<lbm_bvh_node> N(I.nodes.size());
+    for (size_t k = 0; k < I.nodes.size(); ++k) {
+        for (int a = 0; a < 3; ++a) {
+            N[k].lo[a] = I.nodes[k].lo[a];
+            N[k].hi[a] = I.nodes[k].hi[a];
+        }
+        N[k].left = I.nodes[k].left;
+        N[k].right = I.nodes[k].right;
+        N[k].first = I.nodes[k].first;
+        N[k].count = I.nodes[k].count;
+    }
+    const std::vector<int32_t> O(I.order.begin(), I.order.end());
+    const double q[3] = {pnt_ref.x, pnt_ref.y, pnt_ref.z};
+    return lbm_voxelize(V.data(), (int32_t)I.V.size(), T.data(), (int32_t)I.T.size(), N.data(), (int32_t)N.size(),
+                        O.data(), dims, additional, resolution, bbmin, q, grid);
+}
+
 // Grid size in float arithmetic (geometryMesherOFF.h:45-79).
 void geometryMesherOFF::initialization(bool verbose)
 {
@@ -318,6 +345,14 @@ void geometryMesherOFF::meshingPolyhedron(const geometryHandlerOFF& handler, d3 
     const int nx = Constants.dimX_int, ny = Constants.dimY_int, nz = Constants.dimZ_int;
     const double add = Constants.additional_factor, res = Constants.resolution_factor;
     const double bx = Constants.BBminx, by = Constants.BBminy, bz = Constants.BBminz;
+    if (threads < 0) {
+        // the same tests on the GPU; the grid is i fastest, like Data.mesh
+        const int32_t dims[3] = {nx, ny, nz};
+        const double bb[3] = {bx, by, bz};
+        device_status = handler.insideGridDevice(dims, add, res, bb, pnt_outside, Data.meshFluidHost.data());
+        if (verbose) std::cout << "Number of lattice points is " << (long)nx * ny * nz << "\n";
+        return;
+    }
 #ifdef _OPENMP
     const int nth = threads > 0 ? threads : omp_get_max_threads();
 #pragma omp parallel for collapse(2) schedule(dynamic, 4) num_threads(nth)

@@ -157,6 +157,10 @@ public:
     void polyhedronBbox(bool verbose);
     void writeToConstants(LBMConstants& Constants) const;
     bool polyhedronInside(d3 pnt_ask, d3 pnt_ref) const;
+    // polyhedronInside for every cell centre of a grid, on the GPU (lbm_voxelize)
+    // with this handler's hierarchy; grid[cell] = 1 inside.  Returns an lbm_status.
+    int insideGridDevice(const int32_t dims[3], double additional, double resolution, const double bbmin[3],
+                         d3 pnt_ref, int32_t* grid) const;
     int error = 0;
     d3 d3_min_BB{0, 0, 0}, d3_max_BB{0, 0, 0};
     size_t triangles() const;
@@ -172,7 +176,9 @@ class geometryMesherOFF {
 public:
     geometryMesherOFF(LBMConstants& C, LBMData& D) : Constants(C), Data(D) {}
     void initialization(bool verbose);
+    // threads: CPU threads (0 = all); < 0 runs the inside tests on the GPU
     void meshingPolyhedron(const geometryHandlerOFF& handler, d3 pnt_outside, bool verbose, int threads = 0);
+    int device_status = 0;   // lbm_status of the last GPU run
 
 private:
     LBMConstants& Constants;

@@ -74,9 +74,12 @@ def _i32(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
 
 
-def voxelize_off(path, resolution, additional, outside, threads=0):
-    """0/1 grid shaped (nz, ny, nx) plus (bbmin, bbmax) as the reference preprocess computes them."""
+def voxelize_off(path, resolution, additional, outside, threads=0, device=False):
+    """0/1 grid shaped (nz, ny, nx) plus (bbmin, bbmax) as the reference preprocess computes them.
+    device=True runs the inside tests on the GPU (same grid)."""
     L = load_host()
+    if device:
+        threads = -1
     dims = (ctypes.c_int32 * 3)()
     mn, mx = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
     st = L.lbmh_voxelize_off(path.encode(), resolution, additional, _d3(outside), threads, dims, mn, mx, None)

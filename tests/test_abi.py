@@ -99,3 +99,39 @@ def test_valid_descriptor_without_device_reports_hip_error():
     assert _create_status(make_desc(ones, [], tau=0.6)) == abi.LBM_E_HIP
     with pytest.raises(LbmError):
         Lattice(make_desc(flags, bc, tau=0.6))
+
+
+def test_voxelize_rejects_malformed_hierarchies_before_any_device_work():
+    """lbm_voxelize checks every index the traversal will use, and that the
+    hierarchy is a tree laid out parent-first (so every walk ends), on the host."""
+    lib = abi.load_core()
+
+    class Node(ctypes.Structure):
+        _fields_ = [("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3), ("left", ctypes.c_int32),
+                    ("right", ctypes.c_int32), ("first", ctypes.c_int32), ("count", ctypes.c_int32)]
+
+    V = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1], np.float64)
+    T = np.array([0, 1, 2, 0, 1, 3, 0, 2, 3, 1, 2, 3], np.int32)
+    order = np.arange(4, dtype=np.int32)
+    dims = (ctypes.c_int32 * 3)(2, 2, 2)
+    bb = (ctypes.c_double * 3)(0, 0, 0)
+    out = (ctypes.c_double * 3)(5, 5, 5)
+    grid = np.zeros(8, np.int32)
+    i32 = ctypes.POINTER(ctypes.c_int32)
+
+    def call(nodes, tris=T):
+        arr = (Node * len(nodes))(*nodes)
+        return lib.lbm_voxelize(abi.ptr(V), 4, tris.ctypes.data_as(i32), 4, ctypes.cast(arr, ctypes.c_void_p),
+                                len(nodes), order.ctypes.data_as(i32), dims, 1.0, 1.0, bb, out,
+                                grid.ctypes.data_as(i32))
+
+    box = ((ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1))
+    leaf = lambda f, c: Node(box[0], box[1], -1, -1, f, c)   # noqa: E731
+    # a child pointing back at its parent, shared children, a leaf past the triangles
+    assert call([Node(box[0], box[1], 1, 2, 0, 0), Node(box[0], box[1], 0, 2, 0, 0), leaf(0, 4)]) == abi.LBM_E_ARG
+    assert call([Node(box[0], box[1], 1, 1, 0, 0), leaf(0, 4)]) == abi.LBM_E_ARG
+    assert call([Node(box[0], box[1], 1, 2, 0, 0), leaf(0, 2), Node(box[0], box[1], 1, 1, 0, 0)]) == abi.LBM_E_ARG
+    assert call([leaf(2, 3)]) == abi.LBM_E_ARG
+    bad = T.copy()
+    bad[5] = 9
+    assert call([leaf(0, 4)], bad) == abi.LBM_E_ARG
