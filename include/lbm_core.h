@@ -97,7 +97,8 @@ enum {
     LBM_BC_PERIODIC_GHOST = 4,       /* Periodic.h:43-138                            */
     LBM_BC_OUTLET_EQUILIBRIUM = 5,   /* OutletDensityEquilibrium.h:55-76             */
     LBM_BC_OUTLET_NEIGHBOUR_EQ = 6,  /* OutletNeighbourEquilibrium.h:78-126          */
-    LBM_BC_OUTLET_NEIGHBOUR_EQ_RF = 7 /* OutletNeighbourEquilibriumOmegaRF.h:143-198 */
+    LBM_BC_OUTLET_NEIGHBOUR_EQ_RF = 7, /* OutletNeighbourEquilibriumOmegaRF.h:143-198 */
+    LBM_BC_INLET_ZOU_HE = 8          /* InletVelocityZouHe.h:80-101                  */
 };
 
 /* One boundary-list entry; mirrors the reference PODs boundaryConditionWall /

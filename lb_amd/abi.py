@@ -27,7 +27,7 @@ ARITH_LITERAL, ARITH_FAST = 0, 1
 COMPAT_CREV20, COMPAT_F32CONST, COMPAT_LIM210, COMPAT_TRACE17 = 0x1, 0x2, 0x4, 0x8
 COMPAT_REFERENCE = COMPAT_CREV20 | COMPAT_F32CONST | COMPAT_LIM210 | COMPAT_TRACE17
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
-BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF = 5, 6, 7
+BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF, BC_INLET_ZOU_HE = 5, 6, 7, 8
 
 STORAGE_BYTES = {STORAGE_F64: 8, STORAGE_F32: 4, STORAGE_F16DEV: 2, STORAGE_BF16DEV: 2, STORAGE_F16DEV_SR: 2,
                  STORAGE_BF16DEV_SR: 2}

@@ -472,6 +472,7 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
             wall[i] = 1;
             break;
         case LBM_BC_INLET_MOVING_WALL:
+        case LBM_BC_INLET_ZOU_HE:
         case LBM_BC_INLET_EQUILIBRIUM: {
             if (F != -1 || inlet_of[i] >= 0) return LBM_E_ARG;
             NodeDev e = node_dev(L, b);

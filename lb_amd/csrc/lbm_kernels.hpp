@@ -281,6 +281,28 @@ __device__ __forceinline__ void moving_wall(const Consts<R>& K, const NodeDev& b
     }
 }
 
+// InletVelocityZouHe (InletVelocityZouHe.h:80-101): for every outgoing
+// direction q (n.c_q > 0), state[c_rev q] = (f*(q) - f_eq(q; rho, u)) +
+// f_eq(c_rev q; rho, u_inlet).  The inlet velocity is a float vector in the
+// reference, so that equilibrium's polynomial is float.
+template <class R>
+__device__ __forceinline__ void zou_he(const Consts<R>& K, const NodeDev& b, bool crev20, const R* post, R rho, R ux,
+                                       R uy, R uz, R* state)
+{
+    LBM_FP_STRICT
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const float nc = b.nx * (float)CX[q] + b.ny * (float)CY[q] + b.nz * (float)CZ[q];
+        if (nc > 0.0f) {
+            const R fneq = post[q] - feq_q(K, q, ux, uy, uz, rho);
+            const int r = crev_of(q, crev20);
+            const float uc = (float)CX[r] * b.ux + (float)CY[r] * b.uy + (float)CZ[r] * b.uz;
+            const float u2 = b.ux * b.ux + b.uy * b.uy + b.uz * b.uz;
+            state[r] = fneq + K.w[WCLASS[r]] * rho * (R)(1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * u2);
+        }
+    }
+}
+
 template <class R>
 __device__ __forceinline__ R fast_rcp(R x)
 {
@@ -334,11 +356,15 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
             store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
     if (FLAGS && cls != CL_FLUID) {
         if (inlet) {
-            if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
+            const int kind = a.inl[bi].kind;
+            if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
                 R st[Q];
 #pragma unroll
                 for (int q = 0; q < Q; ++q) st[q] = f[q];
-                moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+                if (kind == LBM_BC_INLET_MOVING_WALL)
+                    moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+                else
+                    zou_he(a.K, a.inl[bi], a.crev20 != 0, P, rho, ux, uy, uz, st);
 #pragma unroll
                 for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
             }
@@ -500,11 +526,15 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                 store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
         if (FLAGS && cls != CL_FLUID) {
             if (inlet) {
-                if (a.inl[bi].kind == LBM_BC_INLET_MOVING_WALL) {
+                const int kind = a.inl[bi].kind;
+                if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
                     R st[Q];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) st[q] = f[q];
-                    moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+                    if (kind == LBM_BC_INLET_MOVING_WALL)
+                        moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
+                    else
+                        zou_he(a.K, a.inl[bi], a.crev20 != 0, P, rho, ux, uy, uz, st);
 #pragma unroll
                     for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
                 }

@@ -499,6 +499,20 @@ static void FN(op_inlet)(FN(state) * s)
                                             (ORC_C[vel][0] * b->ux + ORC_C[vel][1] * b->uy + ORC_C[vel][2] * b->uz);
                 }
             }
+        } else if (b->kind == LBM_BC_INLET_ZOU_HE) {
+            /* InletVelocityZouHe::inlet (InletVelocityZouHe.h:80-101): for every
+             * outgoing v (n.c > 0), slot c_rev[v] = (f*(v) - f_eq(v; rho, u)) +
+             * f_eq(c_rev[v]; rho, u_inlet), the inlet velocity's polynomial in float */
+            for (int vel = 0; vel < 27; ++vel) {
+                if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] > 0) {
+                    REAL fNeqRev = DP(s, vel, i) - FN(feq)(s->w, vel, s->u[3 * i], s->u[3 * i + 1],
+                                                          s->u[3 * i + 2], s->rho[i]);
+                    const int r = s->crev[vel];
+                    float uc = ORC_C[r][0] * b->ux + ORC_C[r][1] * b->uy + ORC_C[r][2] * b->uz;
+                    float u2 = b->ux * b->ux + b->uy * b->uy + b->uz * b->uz;
+                    DF(s, r, i) = fNeqRev + s->w[r] * s->rho[i] * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * u2);
+                }
+            }
         } else if (b->kind == LBM_BC_INLET_EQUILIBRIUM) {
             const REAL uxb = b->ux, uyb = b->uy, uzb = b->uz;
             long i1, i2;

@@ -13,7 +13,7 @@ import numpy as np
 
 # lbm_bc_node kinds (include/lbm_core.h)
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
-BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF = 5, 6, 7
+BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF, BC_INLET_ZOU_HE = 5, 6, 7, 8
 
 
 def meshing_boundary_wall(grid: np.ndarray):

@@ -100,3 +100,39 @@ def test_sponge_needs_an_outlet_and_an_axis_normal(gpu):
     with pytest.raises(LbmError):
         Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F32,
                           arith=abi.ARITH_FAST, periodic=(1, 1, 1), tau=0.6, sponge=(0.0, 0.05)))
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_F64, abi.STORAGE_F32], ids=["f64", "f32"])
+def test_zou_he_inlet_literal_equals_oracle(gpu, storage):
+    flags, bcd = mo.channel(24, 12, 10, inlet_kind=abi.BC_INLET_ZOU_HE)
+    bc = bc_nodes(bcd)
+    kw = dict(collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=storage,
+              arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    g = Lattice(make_desc(flags, bc, **kw))
+    o = OracleLattice(make_desc(flags, bc, **kw))
+    solid = flags == 0
+    rng = np.random.default_rng(23)
+    init = (None, np.where(solid, 0.0, 0.03 + 0.004 * rng.standard_normal(flags.shape)), None, None)
+    g.init_equilibrium(*init)
+    o.init_equilibrium(*init)
+    g.step(100)
+    o.step(100)
+    fg, fo = g.populations(), o.populations()
+    print(f"[zou-he storage={storage}] max|df|={np.abs(fg - fo).max():.3e}", flush=True)
+    assert np.array_equal(fg, fo)
+
+
+def test_zou_he_inlet_fast_vs_fp64_oracle(gpu):
+    flags, bcd = mo.channel(24, 12, 10, inlet_kind=abi.BC_INLET_ZOU_HE)
+    bc = bc_nodes(bcd)
+    kw = dict(collision=abi.COLLISION_COMBINED, closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17,
+              compat=abi.COMPAT_REFERENCE, tau=TAU, nu=NU)
+    g = Lattice(make_desc(flags, bc, storage=abi.STORAGE_F32, arith=abi.ARITH_FAST, **kw))
+    o = OracleLattice(make_desc(flags, bc, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, **kw))
+    init = (None, np.where(flags == 0, 0.0, 0.03), None, None)
+    g.init_equilibrium(*init)
+    o.init_equilibrium(*init)
+    g.step(100)
+    o.step(100)
+    ef, er, eu = _errors("zou-he fast f32 combined", g, o, flags)
+    assert eu <= 2e-5 and er <= 2e-5

@@ -517,3 +517,26 @@ def test_outlet_omega_damping_is_a_ramp_over_the_last_tenth():
     expect = np.broadcast_to(1 / tau - ramp, flags.shape)
     assert m == 3
     assert np.allclose(w[fluid], expect[fluid], rtol=0, atol=1e-15)
+
+
+def test_zou_he_inlet_keeps_a_uniform_inflow():
+    """InletVelocityZouHe: incoming = (f* - f_eq)(outgoing) + f_eq(incoming; rho, u_in).
+    A uniform equilibrium flow at the inlet velocity is a fixed point up to
+    the float evaluation of the inlet equilibrium's polynomial (~1e-8)."""
+    u0 = 0.04
+    flags, _, bc = _channel(12, 6, 6, u_in=u0, obstacle=False, walls=False, inlet_kind=abi.BC_INLET_ZOU_HE)
+    o = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    u = np.float64(np.float32(u0))
+    o.init_equilibrium(None, np.full(flags.shape, u), None, None)
+    o.step(20)
+    expect = feq(1.0, (u, 0.0, 0.0))
+    err = np.abs(o.populations() - expect[:, None, None, None]).max()
+    assert 0 < err < 1e-8
+    # a mismatched inflow is pulled towards the inlet velocity at the inlet cells
+    o2 = OracleLattice(make_desc(flags, bc, periodic=(0, 1, 0), collision=abi.COLLISION_CUM2015,
+                                 storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=0, tau=0.6, nu=0.1 / 3))
+    o2.init_equilibrium(None, 0.0, None, None)
+    o2.step(30)
+    inlet = flags == -1
+    assert np.all(o2.macroscopic()[1][inlet] > 0.25 * u0)
