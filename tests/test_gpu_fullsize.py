@@ -21,9 +21,9 @@ pytestmark = pytest.mark.gpu
 SHIFT = (3, 2, 1)   # (z, y, x)
 
 
-def _lattice(n, storage, **kw):
+def _lattice(n, storage, collision=abi.COLLISION_CUM2015, **kw):
     flags = np.ones((n, n, n), np.int32)
-    return Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
+    return Lattice(make_desc(flags, [], collision=collision, closure=abi.CLOSURE_SMAGORINSKY,
                              c_les=0.17, storage=storage, arith=abi.ARITH_FAST, periodic=(1, 1, 1), **kw))
 
 
@@ -91,6 +91,35 @@ def test_conservation_fp64_256(gpu):
     lat.step(10)
     m1 = lat.conservation()
     print(f"[conservation 256^3 f64] mass {m0[0]:.17g} -> {m1[0]:.17g}, "
+          f"momentum drift {np.abs(m1[1:] - m0[1:]).max():.3e}", flush=True)
+    assert abs(m1[0] - m0[0]) <= 1e-12 * m0[0]
+    assert np.all(np.abs(m1[1:] - m0[1:]) <= 1e-12 * m0[0])
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_F32, abi.STORAGE_F16DEV], ids=["f32", "f16dev"])
+def test_translation_invariance_256_cumulant_2017(gpu, storage):
+    """The 2017 operator's limiters and rate functions act per cell: shifted
+    in, shifted out, bit for bit."""
+    n = 256
+    ux, uy, uz = decaying_turbulence(n, k0=4.0, u_rms=0.04, seed=20240602)
+    a, b = _shifted_runs(n, storage, (None, ux, uy, uz), 6, compat=abi.COMPAT_REFERENCE, tau=0.5015,
+                         collision=abi.COLLISION_CUM2017)
+    for k in range(4):
+        assert np.array_equal(np.roll(a[k], SHIFT, axis=(0, 1, 2)), b[k]), k
+
+
+def test_conservation_fp64_256_cumulant_2017(gpu):
+    n = 256
+    ux, uy, uz = decaying_turbulence(n, k0=4.0, u_rms=0.04, seed=20240603)
+    lat = Lattice(make_desc(np.ones((n, n, n), np.int32), [], collision=abi.COLLISION_CUM2017,
+                            closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=abi.STORAGE_F64,
+                            arith=abi.ARITH_FAST, compat=0, periodic=(1, 1, 1), tau=0.5015))
+    lat.init_equilibrium(None, ux, uy, uz)
+    lat.step(1)
+    m0 = lat.conservation()
+    lat.step(10)
+    m1 = lat.conservation()
+    print(f"[conservation 256^3 f64 cum2017] mass drift {abs(m1[0] - m0[0]):.3e}, "
           f"momentum drift {np.abs(m1[1:] - m0[1:]).max():.3e}", flush=True)
     assert abs(m1[0] - m0[0]) <= 1e-12 * m0[0]
     assert np.all(np.abs(m1[1:] - m0[1:]) <= 1e-12 * m0[0])
