@@ -98,6 +98,18 @@ int lbmh_case_run(lbmh_case* c);
 /* Flags of the case and its device lattice (for lbm_get_* reads). */
 int lbmh_case_flags(lbmh_case* c, int32_t* flags);
 lbm_ctx* lbmh_case_lattice(lbmh_case* c);
+/* InitializationEquilibriumVariables (InitializationEquilibriumVariables.h:24-178):
+ * lbmh_read_vtk_fields reads rho (after the first "LOOKUP_TABLE default" line) and u
+ * (after the first "VECTORS U double" line) for dims[0]*dims[1]*dims[2] cells, x
+ * fastest, each value parsed into float, lattice units as written by
+ * variablesLatticeVTK; any output may be NULL.  Returns 0, 1 (cannot open), 2 (rho
+ * short), 3 (u short), 4 (bad arguments).  lbmh_case_set_init_file makes
+ * lbmh_case_initialize start from such a file (name relative to the results
+ * directory) instead of the uniform state; inlet cells still get the inlet
+ * velocity. */
+int lbmh_read_vtk_fields(const char* path, const int32_t dims[3], double* rho, double* ux, double* uy, double* uz);
+int lbmh_case_set_init_file(lbmh_case* c, const char* name);
+
 /* Error norms recorded by the loop (ErrorQuadratic), oldest first. */
 int lbmh_case_errors(lbmh_case* c, double* out, int32_t* n);
 /* Time averages of rho, ux, uy, uz (lattice units, x fastest) that the run

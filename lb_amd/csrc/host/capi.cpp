@@ -246,6 +246,26 @@ int lbmh_case_get(lbmh_case* c, const char* name, double* value)
     return 0;
 }
 
+int lbmh_read_vtk_fields(const char* path, const int32_t dims[3], double* rho, double* ux, double* uy, double* uz)
+{
+    if (!path || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return 4;
+    std::vector<double> r, x, y, z;
+    const int st = readInitFields(path, dims[0], dims[1], dims[2], r, x, y, z);
+    if (st != 0) return st;
+    const std::vector<double>* src[4] = {&r, &x, &y, &z};
+    double* dst[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) std::copy(src[k]->begin(), src[k]->end(), dst[k]);
+    return 0;
+}
+
+int lbmh_case_set_init_file(lbmh_case* c, const char* name)
+{
+    if (!c || !name) return 1;
+    c->C.InitFileName = name;
+    return 0;
+}
+
 int lbmh_case_set_results_dir(lbmh_case* c, const char* dir)
 {
     if (!c || !dir) return 1;

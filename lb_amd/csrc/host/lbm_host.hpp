@@ -74,6 +74,7 @@ struct LBMConstants {
     RealType resolution_factor = 1, additional_factor = 1;
     d3 point_outside{-1, -1, -1};
     std::string file_name, mesh_name;
+    std::string InitFileName;   // InitializationEquilibriumVariables: fields file under results_dir
     int inlet_num = 0, outlet_num = 0, symmetry_num = 0, periodic_num = 0, wall_num = 0;
     // physical
     RealType ny_fyz = 0, u_guess_fyz = 0, L_fyz = 0, rho_fyz = 1, Re = 0;
@@ -215,6 +216,15 @@ private:
     LBMConstants& Constants;
     LBMData& Data;
 };
+
+// The field reader of InitializationEquilibriumVariables
+// (initializations/InitializationEquilibriumVariables.h:57-116): after the first
+// line containing "LOOKUP_TABLE default", nx*ny*nz rho values, then after the
+// first line containing "VECTORS U double", nx*ny*nz velocity triples, x
+// fastest, each parsed into RealType.  Returns 0, 1 (cannot open), 2 (rho
+// short), 3 (u short).
+int readInitFields(const std::string& path, int nx, int ny, int nz, std::vector<double>& rho,
+                   std::vector<double>& ux, std::vector<double>& uy, std::vector<double>& uz);
 
 // outputerMesh (postprocesors/outputerMesh.h:10-213)
 struct outputerMesh {

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 
 #include "lbm_host.hpp"
@@ -133,17 +134,28 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
     }
     int st = lbm_create(&d, &Data.ctx);
     if (st != LBM_OK) return st;
-    // rho = rho_fyz / Cm * Cl^3 and u = u_0 * Cu^-1 on non-solid cells, inlet
-    // velocity on inlet cells (:74-109)
     const size_t N = Data.meshFluidHost.size();
     std::vector<double> rho(N, 0.0), ux(N, 0.0), uy(N, 0.0), uz(N, 0.0);
-    const RealType rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
-    for (size_t i = 0; i < N; ++i) {
-        if (Data.meshFluidHost[i] == 0) continue;
-        rho[i] = rho0;
-        ux[i] = (RealType)(Constants.VelocityInit.x * Constants.Cu_inverse);
-        uy[i] = (RealType)(Constants.VelocityInit.y * Constants.Cu_inverse);
-        uz[i] = (RealType)(Constants.VelocityInit.z * Constants.Cu_inverse);
+    if (!Constants.InitFileName.empty()) {
+        // InitializationEquilibriumVariables (:24-178): rho and u from a VTK
+        // file in lattice units, inlet velocity on inlet cells
+        const int rs = readInitFields(Constants.results_dir + Constants.InitFileName, Constants.dimX_int,
+                                      Constants.dimY_int, Constants.dimZ_int, rho, ux, uy, uz);
+        if (rs != 0) {
+            std::cerr << "Error reading the initial fields (" << rs << ")." << std::endl;
+            return LBM_E_ARG;
+        }
+    } else {
+        // InitializationEquilibriumConstVector: rho = rho_fyz / Cm * Cl^3 and
+        // u = u_0 * Cu^-1 on non-solid cells, inlet velocity on inlet cells (:74-109)
+        const RealType rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
+        for (size_t i = 0; i < N; ++i) {
+            if (Data.meshFluidHost[i] == 0) continue;
+            rho[i] = rho0;
+            ux[i] = (RealType)(Constants.VelocityInit.x * Constants.Cu_inverse);
+            uy[i] = (RealType)(Constants.VelocityInit.y * Constants.Cu_inverse);
+            uz[i] = (RealType)(Constants.VelocityInit.z * Constants.Cu_inverse);
+        }
     }
     for (const auto& b : Data.meshBoundaryInletHost) {
         const size_t i = Data.index(b.vertex.x, b.vertex.y, b.vertex.z);
@@ -155,6 +167,36 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
     u_error.assign(3 * N, 0.0);
     if (verbose) std::cout << "Initialization of simulation done." << std::endl;
     return st;
+}
+
+int readInitFields(const std::string& path, int nx, int ny, int nz, std::vector<double>& rho,
+                   std::vector<double>& ux, std::vector<double>& uy, std::vector<double>& uz)
+{
+    std::ifstream in(path);
+    if (!in.is_open()) return 1;
+    const size_t N = (size_t)nx * ny * nz;
+    rho.assign(N, 0.0);
+    ux.assign(N, 0.0);
+    uy.assign(N, 0.0);
+    uz.assign(N, 0.0);
+    std::string line;
+    while (std::getline(in, line))
+        if (line.find("LOOKUP_TABLE default") != std::string::npos) break;
+    for (size_t i = 0; i < N; ++i) {
+        RealType v;
+        if (!(in >> v)) return 2;
+        rho[i] = v;
+    }
+    while (std::getline(in, line))
+        if (line.find("VECTORS U double") != std::string::npos) break;
+    for (size_t i = 0; i < N; ++i) {
+        RealType a, b, c;
+        if (!(in >> a >> b >> c)) return 3;
+        ux[i] = a;
+        uy[i] = b;
+        uz[i] = c;
+    }
+    return 0;
 }
 
 // probeWrite (:150-177): the probe cell's rho, u are recorded on the device

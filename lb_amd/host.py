@@ -53,6 +53,8 @@ def load_host() -> ctypes.CDLL:
             "lbmh_case_lattice": (ctypes.c_void_p, [vp]),
             "lbmh_case_errors": (ctypes.c_int, [vp, d3, i32p]),
             "lbmh_case_time_avg": (ctypes.c_int, [vp, d3, d3, d3, d3, ctypes.POINTER(ctypes.c_int64)]),
+            "lbmh_read_vtk_fields": (ctypes.c_int, [ctypes.c_char_p, i32p, d3, d3, d3, d3]),
+            "lbmh_case_set_init_file": (ctypes.c_int, [vp, ctypes.c_char_p]),
             "lbmh_case_destroy": (None, [vp]),
         }
         for k, (r, a) in sig.items():

@@ -109,3 +109,56 @@ def test_case_cavity_run_matches_oracle(gpu, tmp_path):
     print(f"[host case] errors {got} oracle {ref_err}", flush=True)
     assert len(got) >= 2
     assert np.allclose(got, ref_err[:len(got)], rtol=1e-6)
+
+
+def test_case_initialized_from_vtk_fields_matches_oracle(gpu, tmp_path):
+    """InitializationEquilibriumVariables through the case API: the fields file
+    (lattice units) sets rho and u, inlet cells take the lid velocity, and the
+    run equals the oracle started from the same (float-rounded) fields."""
+    from test_host import _vtk_fields_text
+    n, steps, tau = 8, 20, 0.56
+    dims = (n + 2, n + 2, n + 2)
+    rng = np.random.default_rng(31)
+    rho = (1 + 0.01 * rng.standard_normal(dims[::-1])).astype(np.float32).astype(np.float64)
+    u = [(0.02 * rng.standard_normal(dims[::-1])).astype(np.float32).astype(np.float64) for _ in range(3)]
+    (tmp_path / "init.vtk").write_text(_vtk_fields_text(dims, rho, u))
+    L = host.load_host()
+    h = ctypes.c_void_p()
+    assert L.lbmh_case_create(ctypes.byref(h)) == 0
+    try:
+        grid = np.zeros(dims[::-1], np.int32)
+        grid[1:-1, 1:-1, 1:-1] = 1
+        assert L.lbmh_case_set_grid(h, (ctypes.c_int32 * 3)(*dims), grid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    1.0, 1.0, (ctypes.c_float * 3)(0, 0, 0), (ctypes.c_float * 3)(n, n, n)) == 0
+        for k, v in (("tau", tau), ("ny", (tau - 0.5) / 3), ("rho_fyz", 1.0), ("Cm", 1.0), ("Cl", 1.0),
+                     ("Cu_inverse", 1.0), ("iterations", steps), ("err_every_it", -1), ("plot_every_it", -1),
+                     ("probe_every_it", -1), ("collision", abi.COLLISION_CUM2015), ("closure", abi.CLOSURE_NONE),
+                     ("storage", abi.STORAGE_F64), ("arith", abi.ARITH_LITERAL), ("compat", abi.COMPAT_REFERENCE)):
+            assert L.lbmh_case_set(h, k.encode(), float(v)) == 0, k
+        assert L.lbmh_case_set_results_dir(h, str(tmp_path).encode()) == 0
+        assert L.lbmh_case_set_init_file(h, b"init.vtk") == 0
+        assert L.lbmh_case_mesh_wall(h) == 0
+        lid = (ctypes.c_float * 3)(float(np.float32(0.05)), 0.0, 0.0)
+        assert L.lbmh_case_inlet(h, (ctypes.c_double * 3)(-1.0, n - 0.75, -1.0),
+                                 (ctypes.c_double * 3)(n + 1.0, n - 0.25, n + 1.0),
+                                 (ctypes.c_double * 3)(-1.0, n - 0.75, n + 1.0), -1, (ctypes.c_float * 3)(0, 1, 0),
+                                 lid, abi.BC_INLET_MOVING_WALL) == 0
+        assert L.lbmh_case_compile(h) == 0
+        assert L.lbmh_case_initialize(h) == 0
+        assert L.lbmh_case_run(h) == 0
+        ctx = L.lbmh_case_lattice(h)
+        out = [np.empty(dims[::-1]) for _ in range(4)]
+        assert abi.load_core().lbm_get_macroscopic(ctx, *[abi.ptr(a) for a in out]) == 0
+    finally:
+        L.lbmh_case_destroy(h)
+    oflags, bc, ux_lid = cavity_setup(n)
+    solid = oflags == 0
+    lidc = ux_lid != 0
+    o = OracleLattice(make_desc(oflags, bc, collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64,
+                                arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=float(np.float32(tau))))
+    o.init_equilibrium(np.where(solid, 0.0, rho), np.where(lidc, ux_lid, np.where(solid, 0.0, u[0])),
+                       np.where(lidc, 0.0, np.where(solid, 0.0, u[1])), np.where(lidc, 0.0, np.where(solid, 0.0, u[2])))
+    o.step(steps)
+    fluid = ~solid
+    for a, b in zip(out, o.macroscopic()):
+        assert np.array_equal(a[fluid], b[fluid])

@@ -213,3 +213,43 @@ def test_nondimensionalize_reference_arithmetic():
     assert abs(get("Re") - 1000.0) < 1e-3
     assert L.lbmh_case_set(h, b"no_such_field", 1.0) == 1
     L.lbmh_case_destroy(h)
+
+
+def _vtk_fields_text(dims, rho, u, pressure=None):
+    nx, ny, nz = dims
+    lines = ["# vtk DataFile Version 2.0", "LBE variables", "ASCII", "DATASET STRUCTURED_POINTS",
+             f"DIMENSIONS {nx} {ny} {nz}", "ASPECT_RATIO 1 1 1", "ORIGIN 0 0 0", f"POINT_DATA {nx * ny * nz}",
+             "SCALARS rho double 1", "LOOKUP_TABLE default"]
+    lines += [repr(float(v)) for v in rho.ravel()]
+    lines += ["SCALARS pressure double 1", "LOOKUP_TABLE default"]
+    lines += [repr(float(v)) for v in (pressure if pressure is not None else rho / 3).ravel()]
+    lines += ["VECTORS U double"]
+    lines += [f"{float(a)!r} {float(b)!r} {float(c)!r}" for a, b, c in zip(*(x.ravel() for x in u))]
+    return "\n".join(lines) + "\n"
+
+
+def test_vtk_initial_fields_reader(tmp_path):
+    """InitializationEquilibriumVariables' reader: rho after the first
+    LOOKUP_TABLE, u after VECTORS U double (the pressure block between is
+    skipped), x fastest, every value rounded to float; short files fail."""
+    import ctypes
+    dims = (5, 4, 3)
+    rng = np.random.default_rng(3)
+    rho = 1 + 0.01 * rng.standard_normal(dims[::-1])
+    u = [0.03 * rng.standard_normal(dims[::-1]) for _ in range(3)]
+    p = tmp_path / "init.vtk"
+    p.write_text(_vtk_fields_text(dims, rho, u))
+    L = host.load_host()
+    d = (ctypes.c_int32 * 3)(*dims)
+    out = [np.empty(dims[::-1]) for _ in range(4)]
+    assert L.lbmh_read_vtk_fields(str(p).encode(), d, *[abi.ptr(a) for a in out]) == 0
+    for got, want in zip(out, [rho] + u):
+        assert np.array_equal(got, want.astype(np.float32).astype(np.float64))
+    text = p.read_text().splitlines()
+    short_u = tmp_path / "short_u.vtk"
+    short_u.write_text("\n".join(text[:-2]) + "\n")
+    assert L.lbmh_read_vtk_fields(str(short_u).encode(), d, *[abi.ptr(a) for a in out]) == 3
+    short_r = tmp_path / "short_r.vtk"
+    short_r.write_text("\n".join(text[:15]) + "\n")
+    assert L.lbmh_read_vtk_fields(str(short_r).encode(), d, *[abi.ptr(a) for a in out]) == 2
+    assert L.lbmh_read_vtk_fields(str(tmp_path / "none.vtk").encode(), d, *[abi.ptr(a) for a in out]) == 1
