@@ -127,10 +127,11 @@ struct StoreBF16Dev {
     {
         return __uint_as_float((uint32_t)ld<NT>(p, i) << 16) + rest;
     }
+    // round to nearest even in one instruction (v_cvt_pk_bf16_f32)
     __device__ __forceinline__ static uint16_t rne(R v, R rest)
     {
-        const uint32_t b = __float_as_uint(v - rest);
-        return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+        const __bf16 h = (__bf16)(v - rest);
+        return __builtin_bit_cast(uint16_t, h);
     }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = rne(v, rest); }
     template <bool NT>
