@@ -91,6 +91,12 @@ def c4_geometry(workdir, res=1.0, add=0.0):
     t0 = time.time()
     grid, mn, mx = host.voxelize_off(path, res, add, (-3.1416, -2.7183, -1.4142))
     tv = time.time() - t0
+    # the same inside tests on the GPU: same grid
+    t0 = time.time()
+    grid_gpu, _, _ = host.voxelize_off(path, res, add, (-3.1416, -2.7183, -1.4142), device=True)
+    tv_gpu = time.time() - t0
+    if not np.array_equal(grid, grid_gpu):
+        raise RuntimeError("GPU voxelization differs from the host's")
     nz, ny, nx = grid.shape
     m = host.Mesher(grid, res, add, mn)
     m.wall()
@@ -110,7 +116,8 @@ def c4_geometry(workdir, res=1.0, add=0.0):
         near |= np.abs(s - 1) < 2.0 / r.min()
     far = ~near
     mism = int(np.count_nonzero(grid[far].astype(bool) == inside_any[far]))
-    info = {"dims": [nx, ny, nz], "triangles": int(len(f)), "voxelize_s": tv, "fluid_cells": int(grid.sum()),
+    info = {"dims": [nx, ny, nz], "triangles": int(len(f)), "voxelize_s": tv, "voxelize_gpu_s": tv_gpu,
+            "fluid_cells": int(grid.sum()),
             "solid_cells": int(grid.size - grid.sum()), "analytic_mismatch_far": mism,
             "near_surface_cells": int(near.sum()), "wall_nodes": int(sum(1 for b in bc if b.kind == abi.BC_WALL)),
             "inlet_nodes": int(sum(1 for b in bc if b.kind == abi.BC_INLET_EQUILIBRIUM)),
