@@ -168,12 +168,36 @@ __device__ __forceinline__ Cell cell_at(const Geo& g, int x, int y, int z)
 }
 
 // Tiles cover the caller's cells only (halo cells are solid and never
-// collide).  Workgroups go to the XCDs in the hardware's round-robin order,
-// which keeps all of them on one narrow band of each plane.
+// collide).  Workgroups go to the XCDs in the hardware's round-robin order
+// (workgroup b on XCD b mod 8), which keeps all of them on one narrow band of
+// each plane.  Where a row of tiles holds several tiles (ntx > 1), the tiles of
+// one row group are given workgroup ids that are equal mod 8: one XCD then
+// handles the whole row group, and the odd step's x-shifted reads across a
+// tile edge find the neighbour tile's cache lines in the same L2.
+__device__ __forceinline__ int tile_of_block(const Geo& g, int b)
+{
+    if (g.ntx == 1) return b;
+    const int nrg = g.ntiles / g.ntx;   // row groups: (y tile, z) pairs
+    const int full = nrg >> 3;          // complete groups of 8 row groups
+    const int grp = 8 * g.ntx;
+    const int gi = b / grp;
+    int R, k;
+    if (gi < full) {
+        const int rem = b - gi * grp;
+        k = rem >> 3;
+        R = gi * 8 + (rem & 7);
+    } else {
+        const int rem = b - full * grp;
+        R = full * 8 + rem / g.ntx;
+        k = rem % g.ntx;
+    }
+    return R * g.ntx + k;
+}
+
 __device__ __forceinline__ Cell cell_of(const Geo& g)
 {
     Cell c;
-    const int t = (int)blockIdx.x;
+    const int t = tile_of_block(g, (int)blockIdx.x);
     const int tx = t % g.ntx;
     const int rem = t / g.ntx;
     const int ty = rem % g.nty;
