@@ -74,6 +74,17 @@ LBM_HD R sqrt_r(R x)
     return std::sqrt(x);
 }
 
+// Reciprocal for the fast arithmetic: the hardware FP32 reciprocal (within
+// 1 ulp) on the device, a correctly rounded quotient otherwise.
+template <class R>
+LBM_HD R frcp(R x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+#endif
+    return R(1) / x;
+}
+
 // ---------------------------------------------------------------------------
 // Equilibrium and moments, literal operation order.
 // ---------------------------------------------------------------------------
@@ -647,19 +658,19 @@ LBM_HD void cum2017_fast(const Consts<R>& K, const R* fin, const Cen2<R>& c, R m
 
     // rates and limiters (Turbulent2017.h:240-291) with omega_2 = 1 substituted
     const R w1 = omg;
-    const R w3 = -R(8) * (w1 - R(2)) * (R(2) * w1 + R(1)) / (R(8) + w1 * (R(14) - R(7) * w1));
-    const R w4 = R(8) * (w1 - R(2)) * (R(4) * w1 - R(7)) / (R(56) - R(50) * w1 + R(8) * w1 * w1);
-    const R w5 = R(24) * (w1 - R(2)) * (R(2) + w1 * (R(7) - R(3) * w1)) /
-                 (R(16) * w1 * w1 * (w1 - R(6)) - R(2) * w1 * (R(216) + R(5) * w1 * (R(9) * w1 - R(46))) +
-                  (w1 * (R(3) * w1 - R(10)) * (R(15) * w1 - R(28)) - R(48)));
-    const R den = (w1 - R(1)) * (R(2) - R(5) * w1);
-    const R A = (R(4) * w1 * w1 + R(2) * w1 * (w1 - R(6)) + w1 * (R(10) - R(3) * w1) - R(4)) / den;
-    const R B = (R(4) * w1 * (R(9) * w1 - R(16)) - R(4) * w1 * w1 - R(2) * (R(2) + R(9) * w1 * (w1 - R(2)))) /
-                (R(3) * den);
+    const R w3 = -R(8) * (w1 - R(2)) * (R(2) * w1 + R(1)) * frcp(R(8) + w1 * (R(14) - R(7) * w1));
+    const R w4 = R(8) * (w1 - R(2)) * (R(4) * w1 - R(7)) * frcp(R(56) - R(50) * w1 + R(8) * w1 * w1);
+    const R w5 = R(24) * (w1 - R(2)) * (R(2) + w1 * (R(7) - R(3) * w1)) *
+                 frcp(R(16) * w1 * w1 * (w1 - R(6)) - R(2) * w1 * (R(216) + R(5) * w1 * (R(9) * w1 - R(46))) +
+                      (w1 * (R(3) * w1 - R(10)) * (R(15) * w1 - R(28)) - R(48)));
+    const R iden = frcp((w1 - R(1)) * (R(2) - R(5) * w1));
+    const R A = (R(4) * w1 * w1 + R(2) * w1 * (w1 - R(6)) + w1 * (R(10) - R(3) * w1) - R(4)) * iden;
+    const R B = (R(4) * w1 * (R(9) * w1 - R(16)) - R(4) * w1 * w1 - R(2) * (R(2) + R(9) * w1 * (w1 - R(2)))) *
+                (iden * R(1.0 / 3.0));
     const R rl = rho * K.lam;
     auto lim = [rl](R w, R a) {
         const R m = std::fabs(a);
-        return (R(1) - w) * rl / (rl + m);   // 1 - (w + (1 - w) |a| / (rl + |a|))
+        return (R(1) - w) * rl * frcp(rl + m);   // 1 - (w + (1 - w) |a| / (rl + |a|))
     };
     const R p120 = C120 + C102, m120 = C120 - C102;
     const R p210 = C210 + C012, m210 = C210 - C012;
@@ -694,7 +705,7 @@ LBM_HD void cum2017_fast(const Consts<R>& K, const R* fin, const Cen2<R>& c, R m
     const R s110 = om1 * c.kxy, s101 = om1 * c.kxz, s011 = om1 * c.kyz;
 
     // fourth-order cumulants, Eq. G2017 43-48 with omega_6..8 = 1: (1/omega_1 - 1/2) rho A, B
-    const R g = (R(1) / w1 - R(0.5)) * rho;
+    const R g = (frcp(w1) - R(0.5)) * rho;
     const R ga = g * A, gb = -K.c13 * g * B * (-R(3) * w1 * ir);
     const R e43 = K.c23 * ga * (Dxu - R(2) * Dyv + Dzw);
     const R e44 = K.c23 * ga * (Dxu + Dyv - R(2) * Dzw);
