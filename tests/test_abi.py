@@ -86,6 +86,21 @@ def test_validation_rejects_bad_descriptors():
     assert _create_status(make_desc(flags, bc, tau=0.6, storage=6, arith=abi.ARITH_FAST)) == abi.LBM_E_ARG
 
 
+@pytest.mark.parametrize("storage,n_over", [
+    (abi.STORAGE_F64, (1024, 1024, 512)),      # 2^29 cells x 8 B = 4 GiB per plane
+    (abi.STORAGE_F32, (1024, 1024, 1024)),     # 2^30 cells x 4 B
+    (abi.STORAGE_F16DEV, (2048, 1024, 1024)),  # 2^31 cells
+], ids=["f64", "f32", "f16dev"])
+def test_largest_lattice_limit(storage, n_over):
+    """A population plane is addressed with 32-bit byte offsets: lattices whose
+    padded plane reaches 4 GiB (or 2^31 cells) are refused at create, before
+    the flag grid is read (a one-cell flag buffer stands in for it here)."""
+    flags = np.ones((1, 1, 1), np.int32)
+    d = make_desc(flags, [], tau=0.6, storage=storage, arith=abi.ARITH_FAST, periodic=(1, 1, 1))
+    d.nx, d.ny, d.nz = n_over
+    assert _create_status(d) == abi.LBM_E_ARG
+
+
 def test_valid_descriptor_without_device_reports_hip_error():
     n, _ = abi.device_info()
     if n > 0:
