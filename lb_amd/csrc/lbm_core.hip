@@ -37,6 +37,7 @@ struct lbm_ctx {
     Geo g{};
     bool flags_needed = false; // any non-fluid cell (halo included)
     int n_inlet = 0, n_eq = 0, n_outlet = 0, n_ghost = 0, n_pub = 0;
+    int n_upd = 0;                // moving-wall and Zou-He inlet entries
     std::vector<int> ghost_off;  // launch boundaries of the ghost list (dependency levels)
     void* pops = nullptr;
     uint8_t* cls = nullptr;
@@ -47,6 +48,8 @@ struct lbm_ctx {
     NodeDev* outl = nullptr;
     NodeDev* ghost = nullptr;
     void* pub_mac = nullptr;
+    void* inl_post = nullptr;     // hand-over to k_inlet_state: f* and rho, u
+    void* inl_mac = nullptr;
     void* pub_post = nullptr;
     void* macro = nullptr;     // 4 planes, arithmetic type, padded layout
     double* stage = nullptr;   // 5 planes FP64, caller layout
@@ -181,6 +184,14 @@ static lbm_status launch_patches(lbm_ctx* c, int odd_next)
 {
     PatchArgs<ST> a = patch_args<ST>(c);
     a.odd = odd_next;
+    if (c->n_upd) {
+        PatchArgs<ST> u = a;
+        u.nodes = c->inl;
+        u.n = c->n_inlet;
+        u.pub_mac = (const typename ST::R*)c->inl_mac;
+        u.pub_post = (const typename ST::R*)c->inl_post;
+        k_inlet_state<ST><<<blocks_for(u.n), TPB, 0, c->stream>>>(u);
+    }
     if (c->n_eq) {
         a.nodes = c->inl;
         a.n = c->n_inlet;
@@ -238,6 +249,8 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.inl = c->inl;
     a.pub_mac = (R*)c->pub_mac;
     a.pub_post = (R*)c->pub_post;
+    a.inl_post = (R*)c->inl_post;
+    a.inl_mac = (R*)c->inl_mac;
     a.macro = (const R*)c->macro;
     a.partials = c->partials;
     a.ccache = c->ccache;
@@ -257,7 +270,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         ma.mlp = closure_weights(c);
         ma.norm = c->cl_norm;
     }
-    const bool patches = c->n_eq || c->n_outlet || c->n_ghost;
+    const bool patches = c->n_upd || c->n_eq || c->n_outlet || c->n_ghost;
     const dim3 grid((unsigned)c->g.ntiles), block(TPB);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -743,7 +756,7 @@ void lbm_destroy(lbm_ctx* c)
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
-                    c->cl_wb,    c->ccache,   c->avg,   c->probe_buf};
+                    c->cl_wb,    c->ccache,   c->avg,   c->probe_buf, c->inl_post, c->inl_mac};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -820,6 +833,8 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     c->flags_needed = P.flags_needed;
     c->n_inlet = (int)P.inl.size();
     c->n_eq = P.n_eq;
+    for (const NodeDev& e : P.inl)
+        if (e.kind == LBM_BC_INLET_MOVING_WALL || e.kind == LBM_BC_INLET_ZOU_HE) c->n_upd++;
     c->n_outlet = (int)P.outl.size();
     if (d->outlet_sponge) {
         // OutletNeighbourEquilibriumOmega::outletOmega (:18-79): the first outlet
@@ -886,6 +901,8 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->ghost, P.ghost.size() * sizeof(NodeDev));
     if (st == LBM_OK) st = dev_alloc(c, &c->pub_mac, (size_t)c->n_pub * PUB_MAC * rb);
     if (st == LBM_OK) st = dev_alloc(c, &c->pub_post, (size_t)c->n_pub * Q * rb);
+    if (st == LBM_OK) st = dev_alloc(c, &c->inl_post, c->n_upd ? (size_t)c->n_inlet * Q * rb : 0);
+    if (st == LBM_OK) st = dev_alloc(c, &c->inl_mac, c->n_upd ? (size_t)c->n_inlet * PUB_MAC * rb : 0);
     if (st == LBM_OK) st = dev_alloc(c, &c->macro, 4 * N * rb);
     if (st == LBM_OK) st = dev_alloc(c, (void**)&c->stage, 5 * NU * sizeof(double));
     const size_t nparts = std::max((size_t)c->g.ntiles, (size_t)blocks_for(NU));

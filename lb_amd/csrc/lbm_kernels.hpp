@@ -96,6 +96,8 @@ struct StepArgs {
     const uint64_t* node;
     const int32_t* bcidx;          // inlet: state index; CL_PUB: publish slot
     typename ST::R* inl_state;     // [n_inlet][27]
+    typename ST::R* inl_post;      // [n_inlet][27]: moving-wall / Zou-He cells' f* of this step
+    typename ST::R* inl_mac;       // [n_inlet][4]: their rho, ux, uy, uz
     const NodeDev* inl;            // [n_inlet]
     typename ST::R* pub_mac;       // [n_pub][PUB_MAC]: rho, u_x, u_y, u_z
     typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
@@ -242,6 +244,19 @@ __host__ __device__ constexpr bool line_aligned(bool odd, int q) { return !odd |
 __host__ __device__ constexpr int read_plane(bool odd, int q) { return odd ? OPP[q] : q; }
 __device__ __forceinline__ uint32_t read_index(const Cell& c, bool odd, int q) { return odd ? nb(c, OPP[q]) : c.i; }
 
+// The boundary path needs a few pre-collision values after the collision; it
+// reads them again instead of keeping all 27 in registers through it.  Here:
+// incoming population q whose source x - c_q is solid or outside.  Nobody
+// writes its read slot this step (the cell's own write there is masked, since
+// x - c_q is no streaming destination, and solid cells do not run), so the
+// slot still holds the value read at the start.
+template <class ST, bool ODD>
+__device__ __forceinline__ typename ST::R reload_pop(typename ST::T* const* p, const typename ST::R* rest,
+                                                     const Cell& c, int q)
+{
+    return ST::template get<false>(p[read_plane(ODD, q)], read_index(c, ODD, q), rest[WCLASS[q]]);
+}
+
 template <class ST, bool ODD>
 __device__ __forceinline__ void load_pop(typename ST::T* const* p, const typename ST::R* rest, const Cell& c,
                                          typename ST::R* f)
@@ -356,7 +371,7 @@ __device__ __forceinline__ void moment_add(double* avg, size_t nu, size_t iu, do
 // arrays through a helper raised its register count from 48-64 to > 100.)
 template <class ST, bool ODD, bool FLAGS>
 __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c, uint8_t cls, int bi, bool inlet,
-                                            const typename ST::R (&f)[Q], const typename ST::R (&P)[Q],
+                                            const typename ST::R (&P)[Q],
                                             typename ST::R rho, typename ST::R ux, typename ST::R uy,
                                             typename ST::R uz)
 {
@@ -367,57 +382,56 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
     const uint32_t S = (uint32_t)(nw & MASK27);
     const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
     const bool wall = (nw >> 63) != 0;
+    // boundary work first, as in k_step
+    if (FLAGS && cls != CL_FLUID && inlet) {
+        const int kind = a.inl[bi].kind;
+        if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
+            R* m = a.inl_mac + (size_t)bi * PUB_MAC;
+            m[0] = rho;
+            m[1] = ux;
+            m[2] = uy;
+            m[3] = uz;
+            R* dst = a.inl_post + (size_t)bi * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) dst[q] = P[q];
+        }
+    }
+    if (FLAGS && cls != CL_FLUID && !inlet) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if ((S >> q) & 1u) {
+                R v;
+                if (!wall)
+                    v = reload_pop<ST, ODD>(a.p, a.rest, c, q);
+                else if (q == 20)
+                    v = a.crev20 ? P[21] : P[19];
+                else
+                    v = P[OPP[q]];
+                store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
+            }
+        }
+        // data the boundary patch kernels of this step read
+        if (cls == CL_PUB) {
+            a.pub_mac[(size_t)bi * PUB_MAC] = rho;
+            a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
+            a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
+            a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
+            R* dst = a.pub_post + (size_t)bi * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) dst[q] = P[q];
+        }
+    }
     // streaming writes, shared by every lane of the wave (boundary lanes drop
     // the writes their mask skips); an inlet cell streams its pre-collision
-    // state (StreamingAB.h:62-65)
+    // state (StreamingAB.h:62-65), which k_inlet_state replaces only after this kernel
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-            store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
+            store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-            store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
-    if (FLAGS && cls != CL_FLUID) {
-        if (inlet) {
-            const int kind = a.inl[bi].kind;
-            if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
-                R st[Q];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) st[q] = f[q];
-                if (kind == LBM_BC_INLET_MOVING_WALL)
-                    moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
-                else
-                    zou_he(a.K, a.inl[bi], a.crev20 != 0, P, rho, ux, uy, uz, st);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                if ((S >> q) & 1u) {
-                    R v;
-                    if (!wall)
-                        v = f[q];
-                    else if (q == 20)
-                        v = a.crev20 ? P[21] : P[19];
-                    else
-                        v = P[OPP[q]];
-                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
-                }
-            }
-            // data the boundary patch kernels of this step read
-            if (cls == CL_PUB) {
-                a.pub_mac[(size_t)bi * PUB_MAC] = rho;
-                a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
-                a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
-                a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
-                R* dst = a.pub_post + (size_t)bi * Q;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) dst[q] = P[q];
-            }
-        }
-    }
+            store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
 }
 
 template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false>
@@ -537,65 +551,70 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         const uint32_t S = (uint32_t)(nw & MASK27);
         const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
         const bool wall = (nw >> 63) != 0;
+        // Boundary work comes before the streaming writes, so that the
+        // post-collision values die as they are stored: bounce-back and kept
+        // links, the data the patch kernels of this step read, and for
+        // moving-wall and Zou-He inlets the collision their state update
+        // (k_inlet_state, after this kernel) starts from.  None of these
+        // writes touches a slot the streaming writes use.
+        if (FLAGS && cls != CL_FLUID && inlet) {
+            const int kind = a.inl[bi].kind;
+            if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
+                R* m = a.inl_mac + (size_t)bi * PUB_MAC;
+                m[0] = rho;
+                m[1] = ux;
+                m[2] = uy;
+                m[3] = uz;
+                R* dst = a.inl_post + (size_t)bi * Q;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) dst[q] = P[q];
+            }
+        }
+        if (FLAGS && cls != CL_FLUID && !inlet) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if ((S >> q) & 1u) {
+                    R v;
+                    if (!wall)
+                        v = reload_pop<ST, ODD>(a.p, a.rest, c, q);
+                    else if (q == 20)
+                        v = a.crev20 ? P[21] : P[19];
+                    else
+                        v = P[OPP[q]];
+                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
+                    if (wall && a.diag) {
+                        // momentum the fluid loses across this link: the
+                        // outgoing f*_{opp q} leaves, the bounced v enters
+                        const double m = (double)v + (double)P[OPP[q]];
+                        dm[4] -= CX[q] * m;
+                        dm[5] -= CY[q] * m;
+                        dm[6] -= CZ[q] * m;
+                    }
+                }
+            }
+            // data the boundary patch kernels of this step read
+            if (cls == CL_PUB) {
+                a.pub_mac[(size_t)bi * PUB_MAC] = rho;
+                a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
+                a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
+                a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
+                R* dst = a.pub_post + (size_t)bi * Q;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) dst[q] = P[q];
+            }
+        }
         // streaming writes, shared by every lane of the wave (boundary lanes drop
         // the writes their mask skips); an inlet cell streams its pre-collision
-        // state (StreamingAB.h:62-65)
+        // state (StreamingAB.h:62-65), which k_inlet_state replaces only after
+        // this kernel
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
+                store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? f[q] : P[q], hsr);
-        if (FLAGS && cls != CL_FLUID) {
-            if (inlet) {
-                const int kind = a.inl[bi].kind;
-                if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
-                    R st[Q];
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) st[q] = f[q];
-                    if (kind == LBM_BC_INLET_MOVING_WALL)
-                        moving_wall(a.K, a.inl[bi], a.crev20 != 0, P, rho, st);
-                    else
-                        zou_he(a.K, a.inl[bi], a.crev20 != 0, P, rho, ux, uy, uz, st);
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) a.inl_state[(size_t)bi * Q + q] = st[q];
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    if ((S >> q) & 1u) {
-                        R v;
-                        if (!wall)
-                            v = f[q];
-                        else if (q == 20)
-                            v = a.crev20 ? P[21] : P[19];
-                        else
-                            v = P[OPP[q]];
-                        store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
-                        if (wall && a.diag) {
-                            // momentum the fluid loses across this link: the
-                            // outgoing f*_{opp q} leaves, the bounced v enters
-                            const double m = (double)v + (double)P[OPP[q]];
-                            dm[4] -= CX[q] * m;
-                            dm[5] -= CY[q] * m;
-                            dm[6] -= CZ[q] * m;
-                        }
-                    }
-                }
-                // data the boundary patch kernels of this step read
-                if (cls == CL_PUB) {
-                    a.pub_mac[(size_t)bi * PUB_MAC] = rho;
-                    a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
-                    a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
-                    a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
-                    R* dst = a.pub_post + (size_t)bi * Q;
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) dst[q] = P[q];
-                }
-            }
-        }
+                store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
     }
 
     if (a.diag) {
@@ -716,7 +735,7 @@ __global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
             for (int q = 0; q < Q; ++q) P[q] = f[q];
             srt_fast(a.K, P, cp.rho, cp.ux, cp.uy, cp.uz, omega);
         }
-        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, f, P, cp.rho, cp.ux, cp.uy, cp.uz);
+        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, P, cp.rho, cp.ux, cp.uy, cp.uz);
     }
 }
 
@@ -802,6 +821,34 @@ __global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) a.inl_state[(size_t)j * Q + q] = feq_q(a.K, q, uxb, uyb, uzb, rhob);
+}
+
+// State update of moving-wall and Zou-He inlet cells (the rules of
+// InletVelocityMovingWall.h:43-49 and InletVelocityZouHe.h:80-101, applied by
+// moving_wall / zou_he) from the collision the step kernel handed over: the
+// cell's post-collision populations (pub_post here) and its rho, u (pub_mac).
+// Runs after the step kernel, which streamed the previous state.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_inlet_state(PatchArgs<ST> a)
+{
+    using R = typename ST::R;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= a.n) return;
+    const NodeDev b = a.nodes[j];
+    if (b.kind != LBM_BC_INLET_MOVING_WALL && b.kind != LBM_BC_INLET_ZOU_HE) return;
+    R P[Q], st[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        P[q] = a.pub_post[(size_t)j * Q + q];
+        st[q] = a.inl_state[(size_t)j * Q + q];
+    }
+    const R* m = a.pub_mac + (size_t)j * PUB_MAC;
+    if (b.kind == LBM_BC_INLET_MOVING_WALL)
+        moving_wall(a.K, b, a.crev20 != 0, P, m[0], st);
+    else
+        zou_he(a.K, b, a.crev20 != 0, P, m[0], m[1], m[2], m[3], st);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.inl_state[(size_t)j * Q + q] = st[q];
 }
 
 // OutletDensityInterpolated::outlet (OutletDensityInterpolated.h:170-252):
