@@ -371,7 +371,7 @@ __device__ __forceinline__ void moment_add(double* avg, size_t nu, size_t iu, do
 // arrays through a helper raised its register count from 48-64 to > 100.)
 template <class ST, bool ODD, bool FLAGS>
 __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c, uint8_t cls, int bi, bool inlet,
-                                            const typename ST::R (&P)[Q],
+                                            typename ST::R (&P)[Q],
                                             typename ST::R rho, typename ST::R ux, typename ST::R uy,
                                             typename ST::R uz)
 {
@@ -395,6 +395,9 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
 #pragma unroll
             for (int q = 0; q < Q; ++q) dst[q] = P[q];
         }
+        // an inlet cell streams its pre-collision state (StreamingAB.h:62-65)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) P[q] = a.inl_state[(size_t)bi * Q + q];
     }
     if (FLAGS && cls != CL_FLUID && !inlet) {
 #pragma unroll
@@ -422,16 +425,15 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
         }
     }
     // streaming writes, shared by every lane of the wave (boundary lanes drop
-    // the writes their mask skips); an inlet cell streams its pre-collision
-    // state (StreamingAB.h:62-65), which k_inlet_state replaces only after this kernel
+    // the writes their mask skips)
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-            store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
+            store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-            store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
+            store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 }
 
 template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false>
@@ -556,7 +558,8 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         // links, the data the patch kernels of this step read, and for
         // moving-wall and Zou-He inlets the collision their state update
         // (k_inlet_state, after this kernel) starts from.  None of these
-        // writes touches a slot the streaming writes use.
+        // writes touches a slot the streaming writes use.  Inlet cells then
+        // take their previous state into P: that is what they stream.
         if (FLAGS && cls != CL_FLUID && inlet) {
             const int kind = a.inl[bi].kind;
             if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
@@ -569,6 +572,9 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 #pragma unroll
                 for (int q = 0; q < Q; ++q) dst[q] = P[q];
             }
+            // an inlet cell streams its pre-collision state (StreamingAB.h:62-65)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) P[q] = a.inl_state[(size_t)bi * Q + q];
         }
         if (FLAGS && cls != CL_FLUID && !inlet) {
 #pragma unroll
@@ -604,17 +610,15 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             }
         }
         // streaming writes, shared by every lane of the wave (boundary lanes drop
-        // the writes their mask skips); an inlet cell streams its pre-collision
-        // state (StreamingAB.h:62-65), which k_inlet_state replaces only after
-        // this kernel
+        // the writes their mask skips)
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, true>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
+                store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, false>(a.p, a.rest, c, q, inlet ? a.inl_state[(size_t)bi * Q + q] : P[q], hsr);
+                store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
     }
 
     if (a.diag) {
