@@ -53,8 +53,9 @@ def profiled_traffic(storage_name, n):
 
 
 def closure_result():
-    """Dissipation and spectrum errors of the learned closure against the
-    filtered fine run, from the committed experiment summary."""
+    """Dissipation and spectrum errors of the learned closure and its
+    controls against the filtered fine run, quoted from the committed
+    experiment summary (the experiment itself takes minutes)."""
     path = os.path.join(ROOT, "profiles", "closure_c5_512to256.json")
     if not os.path.exists(path):
         return None
@@ -70,6 +71,68 @@ def closure_result():
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+C1_N, C1_STEPS, C1_TAU, C1_LID = 64, 2000, 0.524, 0.05
+
+
+def c1_cavity(n=C1_N):
+    """Lid-driven cavity of the C1 workload, meshed by the host mesher: n^3
+    fluid cells inside a one-cell solid halo, the top fluid layer (y = n) a
+    moving-wall inlet at (0.05, 0, 0) with outward normal +y, half-way
+    bounce-back on the other walls.  Returns (flags, bc, initial u_x)."""
+    from lb_amd import host
+
+    g = np.zeros((n + 2, n + 2, n + 2), np.int32)
+    g[1:-1, 1:-1, 1:-1] = 1
+    m = host.Mesher(g, 1.0, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    # cell centres at j + 0.5 - 1: the slab [n - 0.75, n - 0.25] holds j = n only
+    m.inlet((-1.0, n - 0.75, -1.0), (n + 1.0, n - 0.25, n + 1.0), (-1.0, n - 0.75, n + 1.0), -1, (0.0, 1.0, 0.0),
+            (C1_LID, 0.0, 0.0), abi.BC_INLET_MOVING_WALL)
+    flags, bc = m.compile()
+    ux = np.zeros(flags.shape)
+    for b in bc:
+        if b.kind == abi.BC_INLET_MOVING_WALL:
+            ux[b.z, b.y, b.x] = b.ux
+    return flags, bc, ux
+
+
+def c1_desc(flags, bc):
+    return make_desc(flags, bc, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_NONE, storage=abi.STORAGE_F64,
+                     arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE, tau=C1_TAU)
+
+
+def c1_gpu(flags, bc, ux):
+    """The C1 cavity on the GPU: FP64 storage, literal arithmetic, all steps."""
+    L = Lattice(c1_desc(flags, bc))
+    L.init_equilibrium(None, ux, None, None)
+    L.step(10)
+    L.sync()
+    L.init_equilibrium(None, ux, None, None)
+    t0 = time.perf_counter()
+    L.step(C1_STEPS)
+    L.sync()
+    wall = time.perf_counter() - t0
+    kms = L.last_kernel_ms()
+    m = L.macroscopic()
+    L.close()
+    return wall, kms, m
+
+
+def c1_cpu(flags, bc, ux):
+    """The C1 cavity on the CPU oracle (reference order, FP64), all steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleLattice, oracle_lib
+
+    o = OracleLattice(c1_desc(flags, bc))
+    o.init_equilibrium(None, ux, None, None)
+    t0 = time.perf_counter()
+    o.step(C1_STEPS)
+    dt = time.perf_counter() - t0
+    m = o.macroscopic()
+    o.close()
+    return dt, oracle_lib().orc_get_threads(), m
 
 
 def make_lattice(n, storage, fields, arith=abi.ARITH_FAST, collision=abi.COLLISION_CUM2015):
@@ -152,6 +215,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the other storage formats")
+    ap.add_argument("--no-c1", action="store_true", help="skip the C1 cavity row")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -189,7 +253,7 @@ def main():
     achieved = bytes_per_launch / (res["kernel_ms"] * 1e-3) / 1e9
     traffic, traffic_src = profiled_traffic(args.storage, n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_quoted_from": traffic_src,
                 "algorithmic_bytes_per_launch": bytes_per_launch,
                 "kernel": "k_step (fused collide-stream, A-A)", "bytes_per_cell": 2 * 27 * sbytes,
                 "kernel_ms": round(res["kernel_ms"], 5), "copy_kernel_gbs": round(probe_gbs, 1)}
@@ -234,6 +298,27 @@ def main():
         cpu = cpu_baseline(n, fields, args.cpu_steps, 2)
         log(f"cpu baseline: {cpu['value']} MLUPS on {cpu['cores']} threads")
 
+    # C1: the lid-driven cavity, FP64 literal on the GPU and on the CPU oracle
+    c1 = None
+    if not args.no_c1:
+        flags1, bc1, ux1 = c1_cavity()
+        cells1 = int((flags1 != 0).sum())
+        wall1, kms1, mg = c1_gpu(flags1, bc1, ux1)
+        c1 = {"workload": f"C1 lid-driven cavity {C1_N}^3 (+1 solid halo), Re=400, lid u={C1_LID}, tau={C1_TAU}, "
+                          f"cumulant-2015 laminar, FP64 literal, {C1_STEPS} steps",
+              "cells": cells1, "gpu_mlups": round(cells1 * C1_STEPS / wall1 / 1e6, 1),
+              "gpu_kernel_ms": round(kms1, 5)}
+        if not args.no_cpu_baseline:
+            dt1, cores1, mo = c1_cpu(flags1, bc1, ux1)
+            fl = flags1 != 0
+            num = np.sqrt(sum(np.sum((a[fl] - b[fl]) ** 2) for a, b in zip(mg[1:], mo[1:])))
+            den = np.sqrt(sum(np.sum(b[fl] ** 2) for b in mo[1:]))
+            c1.update({"cpu_mlups": round(cells1 * C1_STEPS / dt1 / 1e6, 2), "cpu_cores": cores1,
+                       "cpu_kind": "port", "gpu_vs_cpu_rel_l2_u": float(num / den)})
+        log(f"C1 cavity: GPU {c1['gpu_mlups']} MLUPS" +
+            (f", CPU {c1['cpu_mlups']} MLUPS on {c1['cpu_cores']} threads, rel-L2(u) "
+             f"{c1['gpu_vs_cpu_rel_l2_u']:.1e}" if "cpu_mlups" in c1 else ""))
+
     out = {
         "metric": METRIC,
         "value": round(res["mlups"], 1),
@@ -254,7 +339,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": variants,
-        "closure_error": closure_result(),
+        "c1": c1,
+        "quoted_from_profiles": {"closure_c5": closure_result()},
     }
     print(json.dumps(out), flush=True)
 

@@ -84,7 +84,18 @@ enum { LBM_ARITH_LITERAL = 0, LBM_ARITH_FAST = 1 };
                                       |C_120 +- C_102| (correct: |C_210 +- C_012|)          */
 #define LBM_COMPAT_TRACE17   0x8u  /* Cum2017.h:295 subtracts (rho - 1), Combined.h:798 (rho + 1)
                                       from the trace in D_x u (correct: k_000, as Cum2015.h:253) */
-#define LBM_COMPAT_REFERENCE (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST | LBM_COMPAT_LIM210 | LBM_COMPAT_TRACE17)
+#define LBM_COMPAT_INLETRHO  0x10u /* InletVelocityEquilibrium.h:80, 95 extrapolate the inlet density
+                                      with nu/3 where the steady x-momentum balance
+                                      d(rho/3)/dx = nu d2u/dx2 gives 3 nu, and the +x branch
+                                      subtracts the second difference its -x branch adds
+                                      (correct: rho_b = rho_1 + 3 nu (x_b - x_1) d2u/dx2)      */
+#define LBM_COMPAT_EQ44      0x20u /* Cum2017.h:351, Combined.h:351 pair C_220 + C_202 - 2 C_022 with
+                                      D_x u + D_y v - 2 D_z w; the axis convention of Eq. 43 (and
+                                      symmetry under x <-> z) gives -2 D_x u + D_y v + D_z w.  The
+                                      reference's form exchanges the A terms of C_220 and C_022 */
+#define LBM_COMPAT_REFERENCE                                                                       \
+    (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST | LBM_COMPAT_LIM210 | LBM_COMPAT_TRACE17 | LBM_COMPAT_INLETRHO | \
+     LBM_COMPAT_EQ44)
 
 /* Boundary-node kinds.  The flag grid keeps the reference encoding
  * (geometryMesherBoundary.h): 0 solid, >0 fluid (1 + number of solid
@@ -209,6 +220,22 @@ lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
  * this is the momentum-exchange sum 2 f*(x, k) c_k over links k into the
  * solid.  (No reference counterpart; the learned closure is excluded.) */
 lbm_status lbm_step_diagnostics(lbm_ctx* ctx, double out[7]);
+
+/* ErrorQuadratic::errorEvaluation (errorEvaluations/ErrorQuadratic.h:33-90),
+ * the solver loop's convergence norm, on the device: with u the velocity
+ * lbm_get_macroscopic returns and u_prev the velocity of the previous call
+ * (zero before the first),
+ *   err = sqrt(sum sqrt((ux - ux_prev)^2 + (uy - uy_prev)^2 + (uy - uz_prev)^2))
+ *       / sqrt(sum sqrt(ux^2 + uy^2 + uz^2))
+ * over cells whose flag is 1 (the third difference keeps the reference's
+ * uy - uz_prev, :43-44); then u_prev = u.  lbm_init_equilibrium sets u_prev
+ * back to zero.  Each cell's terms are formed in the
+ * arithmetic type (FP64 or FP32, the reference's RealType) and summed in
+ * FP64; only the two sums cross to the host.  The first call allocates
+ * u_prev (3 values per cell).  Synchronises.  Returns LBM_E_NAN (and
+ * lbm_sync keeps returning it) when a sum is not a number, which is the
+ * reference loop's break condition (SolverTurbulentLES.h:238-241). */
+lbm_status lbm_error_quadratic(lbm_ctx* ctx, double* err);
 
 /* Time averaging, MomentTimeAvg (moments/MomentTimeAvg.h:19-80) as the
  * solver uses it (SolverTurbulentLES.h:216-221, 255-264).  Running sums of

@@ -224,11 +224,11 @@ def kinetic_energy(u, v, w):
 
 def les_run(filtered0, nu, closure, steps, every, c_les=0.17, params=None, norm=None, refresh=1):
     """Coarse LES from a filtered field: returns (steps, K(t)) and the final
-    velocity, with closure 'smagorinsky' or 'mlp' (network evaluated every
-    `refresh` steps)."""
+    velocity, with closure 'smagorinsky', 'mlp' (network evaluated every
+    `refresh` steps) or 'none' (the molecular rate alone, no sub-grid model)."""
     n = filtered0.shape[1]
     flags = np.ones((n, n, n), np.int32)
-    cl = abi.CLOSURE_MLP if closure == "mlp" else abi.CLOSURE_SMAGORINSKY
+    cl = {"mlp": abi.CLOSURE_MLP, "none": abi.CLOSURE_NONE}.get(closure, abi.CLOSURE_SMAGORINSKY)
     lat = Lattice(make_desc(flags, [], collision=abi.COLLISION_CUM2015, closure=cl, storage=abi.STORAGE_F32,
                             arith=abi.ARITH_FAST, compat=0, periodic=(1, 1, 1), tau=3.0 * nu + 0.5, c_les=c_les))
     if closure == "mlp":
@@ -318,9 +318,14 @@ def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_
     fine_K = np.array(fine_K)
     les_steps = eval_steps // f_eval
     out = {}
-    runs = (("smagorinsky", "smagorinsky", 1), ("mlp", "mlp", 1), (f"mlp_refresh{refresh}", "mlp", refresh))
-    for name, kind, every in runs:
-        t, K, fin, ms, frac = les_run(filt0, nu_e, kind, les_steps, eval_every, c_les, params, norm, every)
+    # controls: the reference's constant, the constant fitted to the training
+    # targets (their dissipation-weighted mean), and no sub-grid model at all
+    c_fit = res["target_weighted_mean_C"]
+    runs = (("smagorinsky", "smagorinsky", 1, c_les), ("smagorinsky_fitted", "smagorinsky", 1, c_fit),
+            ("no_model", "none", 1, c_les), ("mlp", "mlp", 1, c_les),
+            (f"mlp_refresh{refresh}", "mlp", refresh, c_les))
+    for name, kind, every, c_run in runs:
+        t, K, fin, ms, frac = les_run(filt0, nu_e, kind, les_steps, eval_every, c_run, params, norm, every)
         eps = dissipation(t, K)
         eps_ref = dissipation(fine_t, fine_K)
         # compare after the initial transient (first tenth of the run)
@@ -331,7 +336,7 @@ def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_
         kmax = len(Ef) // 2
         spec = float(np.sqrt(np.mean((np.log(El[1:kmax]) - np.log(Ef[1:kmax])) ** 2)))
         out[name] = {"dissipation_rel_err": rel, "spectrum_log_rms": spec, "ms_per_step": ms,
-                     "K_final": float(K[-1]), "in_envelope": frac}
+                     "K_final": float(K[-1]), "in_envelope": frac, "c_les": c_run if kind != "none" else None}
         _log(f"{name}: dissipation rel err {rel:.3f}, spectrum log-rms {spec:.3f}, {ms:.3f} ms/step"
              + (f", envelope {frac:.3f}" if frac is not None else ""))
     # the same network with wider training envelopes (fewer Smagorinsky fallbacks)

@@ -155,6 +155,13 @@ class Lattice:
         _check(self._lib.lbm_step_diagnostics(self._h, abi.ptr(out)), "lbm_step_diagnostics")
         return out[0], out[1:4].copy(), out[4:7].copy()
 
+    def error_quadratic(self) -> float:
+        """ErrorQuadratic's convergence norm against the previous call's
+        velocity (reduced on the device); NaN sums raise LBM_E_NAN."""
+        v = ctypes.c_double()
+        _check(self._lib.lbm_error_quadratic(self._h, ctypes.byref(v)), "lbm_error_quadratic")
+        return v.value
+
     def steps(self) -> int:
         v = ctypes.c_int64()
         _check(self._lib.lbm_step_count(self._h, ctypes.byref(v)), "lbm_step_count")

@@ -39,6 +39,7 @@ struct Consts {
     R lam;      // limiter parameter lambda_3..5 of Geier 2017 (reference: 0.01f)
     int lim210;   // the 210/012 limiters read |C_120 +- C_102| (reference quirk)
     int trace17;  // 2017 bulk term subtracts rho -+ 1 instead of k_000 (reference quirk)
+    int eq44;     // 2017 Eq. 44 with D_x u + D_y v - 2 D_z w (reference quirk)
 };
 
 // Constants for a set of LBM_COMPAT_* bits.  Float literals of the reference
@@ -60,6 +61,7 @@ inline Consts<R> make_consts(uint32_t compat)
     K.lam = f32 ? (R)0.01f : (R)0.01;
     K.lim210 = (compat & LBM_COMPAT_LIM210) != 0;
     K.trace17 = (compat & LBM_COMPAT_TRACE17) != 0;
+    K.eq44 = (compat & LBM_COMPAT_EQ44) != 0;
     return K;
 }
 
@@ -407,7 +409,8 @@ LBM_HD void cum_literal(const Consts<R>& K, R* f, R rho, R ux, R uy, R uz, R omg
         // fourth order, Eq. G2017 43-48 (:349-365)
         const R e43 = K.c23 * (R(1) / w1 - R(0.5)) * w6 * A * rho * (Dxu - R(2) * Dyv + Dzw) +
                       (R(1) - w6) * (C220 - R(2) * C202 + C022);
-        const R e44 = K.c23 * (R(1) / w1 - R(0.5)) * w6 * A * rho * (Dxu + Dyv - R(2) * Dzw) +
+        const R e44 = K.c23 * (R(1) / w1 - R(0.5)) * w6 * A * rho *
+                          (K.eq44 ? (Dxu + Dyv - R(2) * Dzw) : (Dyv + Dzw - R(2) * Dxu)) +
                       (R(1) - w6) * (C220 + C202 - R(2) * C022);
         const R e45 = -K.c43 * (R(1) / w1 - R(0.5)) * w7 * A * rho * (Dxu + Dyv + Dzw) +
                       (R(1) - w7) * (C220 + C202 + C022);
@@ -708,7 +711,7 @@ LBM_HD void cum2017_fast(const Consts<R>& K, const R* fin, const Cen2<R>& c, R m
     const R g = (frcp(w1) - R(0.5)) * rho;
     const R ga = g * A, gb = -K.c13 * g * B * (-R(3) * w1 * ir);
     const R e43 = K.c23 * ga * (Dxu - R(2) * Dyv + Dzw);
-    const R e44 = K.c23 * ga * (Dxu + Dyv - R(2) * Dzw);
+    const R e44 = K.c23 * ga * (K.eq44 ? (Dxu + Dyv - R(2) * Dzw) : (Dyv + Dzw - R(2) * Dxu));
     const R e45 = -K.c43 * ga * (Dxu + Dyv + Dzw);
     const R c220 = K.c13 * (e43 + e44 + e45), c202 = K.c13 * (e45 - e43), c022 = K.c13 * (e45 - e44);
     const R c211 = gb * c.kyz, c121 = gb * c.kxz, c112 = gb * c.kxy;

@@ -279,7 +279,6 @@ public:
     bool write_output = true;  // VTK at plot_every_it
     Timer timer_loop, timer_collision, timer_streaming, timer_bounceback, timer_momentsUpdate, timer_err,
         timer_output, timer_dumping, timer_timeAvg;
-    std::vector<double> u_error;
     std::vector<double> err_history;
 
 private:

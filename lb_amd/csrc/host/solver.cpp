@@ -164,7 +164,6 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
         uz[i] = b.velocity.z;
     }
     st = lbm_init_equilibrium(Data.ctx, rho.data(), ux.data(), uy.data(), uz.data());
-    u_error.assign(3 * N, 0.0);
     if (verbose) std::cout << "Initialization of simulation done." << std::endl;
     return st;
 }
@@ -223,31 +222,15 @@ int SolverTurbulentLES::probeFlush()
 }
 
 // ErrorQuadratic (ErrorQuadratic.h:33-90) on fluid cells with flag == 1,
-// including the third-component typo (uy - u_prev.z); summed in double.
+// including the third-component typo (uy - u_prev.z): reduced on the device
+// (lbm_error_quadratic keeps u_prev there), so only the two sums cross to the
+// host at the loop's error cadence.
 int SolverTurbulentLES::errorEvaluation()
 {
-    const size_t N = Data.meshFluidHost.size();
-    Data.rho_out.resize(N);
-    Data.ux_out.resize(N);
-    Data.uy_out.resize(N);
-    Data.uz_out.resize(N);
-    const int st = lbm_get_macroscopic(Data.ctx, Data.rho_out.data(), Data.ux_out.data(), Data.uy_out.data(),
-                                       Data.uz_out.data());
+    double e = 0.0;
+    const int st = lbm_error_quadratic(Data.ctx, &e);
     if (st != LBM_OK && st != LBM_E_NAN) return st;
-    double e1 = 0, e2 = 0;
-    for (size_t i = 0; i < N; ++i) {
-        if (Data.meshFluidHost[i] != 1) continue;
-        const double ux = Data.ux_out[i], uy = Data.uy_out[i], uz = Data.uz_out[i];
-        const double dx = ux - u_error[3 * i], dy = uy - u_error[3 * i + 1], dz = uy - u_error[3 * i + 2];
-        e1 += std::sqrt(dx * dx + dy * dy + dz * dz);
-        e2 += std::sqrt(ux * ux + uy * uy + uz * uz);
-    }
-    Constants.err = (RealType)(std::sqrt(e1) / std::sqrt(e2));
-    for (size_t i = 0; i < N; ++i) {
-        u_error[3 * i] = Data.ux_out[i];
-        u_error[3 * i + 1] = Data.uy_out[i];
-        u_error[3 * i + 2] = Data.uz_out[i];
-    }
+    Constants.err = (RealType)e;
     err_history.push_back(Constants.err);
     return st;
 }

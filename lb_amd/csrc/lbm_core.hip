@@ -82,6 +82,12 @@ struct lbm_ctx {
     std::vector<double> probe_host;  // records drained from the device, oldest first
     // outlet omega damping
     int sp_axis = -1, sp_begin = 0, sp_interval = 1;
+    // ErrorQuadratic on the device
+    std::vector<uint8_t> h_flag1;   // flag == 1 per caller cell (the cells the norm sums over)
+    uint8_t* flag1 = nullptr;       // the same on the device, uploaded by the first evaluation
+    void* uerr = nullptr;           // previous velocity, 3 planes of the arithmetic type, caller layout
+    // per-block partials of a diagnostics step go here instead of `partials` when set
+    double* diag_dst = nullptr;
 };
 
 static constexpr int PROBE_CAP = 4096;
@@ -173,6 +179,7 @@ static PatchArgs<ST> patch_args(const lbm_ctx* c)
     // the outlet's interpolation weight (OutletDensityInterpolated.h:226) is
     // 1/sqrt(3) in the constants' precision
     a.cs = (c->d.compat & LBM_COMPAT_F32CONST) ? (R)(1.f / sqrtf(3.f)) : (R)(1.0 / std::sqrt(3.0));
+    a.inletrho = (c->d.compat & LBM_COMPAT_INLETRHO) ? 1 : 0;
     a.crev20 = (c->d.compat & LBM_COMPAT_CREV20) ? 1 : 0;
     return a;
 }
@@ -252,7 +259,7 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     a.inl_post = (R*)c->inl_post;
     a.inl_mac = (R*)c->inl_mac;
     a.macro = (const R*)c->macro;
-    a.partials = c->partials;
+    a.partials = c->diag_dst ? c->diag_dst : c->partials;
     a.ccache = c->ccache;
     a.g = c->g;
     a.sp_axis = c->sp_axis;
@@ -756,7 +763,8 @@ void lbm_destroy(lbm_ctx* c)
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
-                    c->cl_wb,    c->ccache,   c->avg,   c->probe_buf, c->inl_post, c->inl_mac};
+                    c->cl_wb,    c->ccache,   c->avg,   c->probe_buf, c->inl_post, c->inl_mac,
+                    c->flag1,    c->uerr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -861,6 +869,8 @@ lbm_status lbm_create(const lbm_desc* d, lbm_ctx** out)
         c->sp_begin = dim - c->sp_interval;
         c->sp_axis = axis;
     }
+    c->h_flag1.resize(NU);
+    for (size_t i = 0; i < NU; ++i) c->h_flag1[i] = d->flags[i] == 1 ? 1 : 0;
     c->n_ghost = (int)P.ghost.size();
     c->n_pub = P.n_pub;
     c->ghost_off = P.ghost_off;
@@ -962,6 +972,8 @@ lbm_status lbm_init_equilibrium(lbm_ctx* c, const double* rho, const double* ux,
         return LBM_OK;
     });
     if (st != LBM_OK) return st;
+    // the convergence norm's previous velocity starts again from zero
+    if (c->uerr) HIPCHK(hipMemsetAsync(c->uerr, 0, 3 * c->NU * c->rbytes, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->initialized = true;
     c->first = true;
@@ -1398,6 +1410,45 @@ lbm_status lbm_step_diagnostics(lbm_ctx* c, double out[7])
         finite = finite && std::isfinite(s[k]);
     }
     if (!finite) {
+        c->nan_seen = true;
+        return LBM_E_NAN;
+    }
+    return LBM_OK;
+}
+
+lbm_status lbm_error_quadratic(lbm_ctx* c, double* err)
+{
+    if (!c || !err) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    lbm_status st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    if (!c->uerr) {
+        // first evaluation: the previous velocity starts at zero, as the
+        // reference's u_error field and the host loop's copy of it do
+        st = dev_alloc(c, &c->uerr, 3 * c->NU * c->rbytes);
+        if (st == LBM_OK) st = dev_alloc(c, (void**)&c->flag1, c->NU);
+        if (st != LBM_OK) return st;
+        HIPCHK(hipMemsetAsync(c->uerr, 0, 3 * c->NU * c->rbytes, c->stream));
+        HIPCHK(hipMemcpyAsync(c->flag1, c->h_flag1.data(), c->NU, hipMemcpyHostToDevice, c->stream));
+    }
+    const unsigned nb = blocks_for(c->NU);
+    double* res = c->partials + (size_t)nb * 2;   // the partials buffer holds >= 8 doubles per block
+    st = with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        a.io = c->partials;
+        k_error<ST><<<nb, TPB, 0, c->stream>>>(a, (typename ST::R*)c->uerr, c->flag1);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (st != LBM_OK) return st;
+    k_reduce_partials<<<1, TPB, 0, c->stream>>>(c->partials, (int)nb, 2, 2, res);
+    HIPCHK(hipGetLastError());
+    double sums[2];
+    HIPCHK(hipMemcpyAsync(sums, res, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *err = std::sqrt(sums[0]) / std::sqrt(sums[1]);
+    if (std::isnan(sums[0]) || std::isnan(sums[1])) {
         c->nan_seen = true;
         return LBM_E_NAN;
     }

@@ -763,6 +763,7 @@ struct PatchArgs {
     typename ST::R rest[4];
     typename ST::R nu, cs;
     int crev20;
+    int inletrho;   // LBM_COMPAT_INLETRHO: the reference's nu/3 inlet extrapolation
     int odd;   // parity of the step that will read the state
 };
 
@@ -798,6 +799,10 @@ __device__ __forceinline__ void store_node(const PatchArgs<ST>& a, const typenam
 // rho_b is extrapolated from rho and u_x of the first two interior
 // neighbours as they entered this step's collision, and the inlet state
 // becomes f_eq(u_b, rho_b).  Only +-x normals act, as in the reference.
+// With duxb, dux1 the differences each branch of the reference forms,
+// rho_b = rho_1 + 3 nu (duxb - dux1) = rho_1 + 3 nu (x_b - x_1) d2u/dx2 (the
+// steady x-momentum balance d(rho/3)/dx = nu d2u/dx2); the reference's
+// nu/3, and its sign on the +x branch (:80), under LBM_COMPAT_INLETRHO.
 template <class ST>
 __global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
 {
@@ -815,11 +820,11 @@ __global__ __launch_bounds__(TPB) void k_inlet_eq(PatchArgs<ST> a)
     if (b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f) {
         const R duxb = (uxb - ux1);
         const R dux1 = (ux1 - ux2);
-        rhob = rho1 + a.nu / 3 * (dux1 - duxb);
+        rhob = a.inletrho ? rho1 + a.nu / 3 * (dux1 - duxb) : rho1 + R(3) * a.nu * (duxb - dux1);
     } else if (b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f) {
         const R duxb = (ux1 - uxb);
         const R dux1 = (ux2 - ux1);
-        rhob = rho1 + a.nu / 3 * (duxb - dux1);
+        rhob = a.inletrho ? rho1 + a.nu / 3 * (duxb - dux1) : rho1 + R(3) * a.nu * (duxb - dux1);
     } else {
         return;
     }
@@ -1254,6 +1259,83 @@ __global__ void k_probe(AuxArgs<ST> a, int x, int y, int z)
     a.io[1] = (double)ux;
     a.io[2] = (double)uy;
     a.io[3] = (double)uz;
+}
+
+// ErrorQuadratic::errorEvaluation (ErrorQuadratic.h:33-90) of the current
+// state against the velocity uerr (3 planes, caller layout, the arithmetic
+// type like the reference's u_error field): per block, the FP64 sums of
+// sqrt(|u - u_prev|^2) and sqrt(|u|^2) over cells whose flag is 1 (the third
+// difference is u_y - u_prev,z, ErrorQuadratic.h:43-44), written to
+// a.io[2 block + {0, 1}]; then u_prev = u on every cell (0 on solid cells).
+// u is what lbm_get_macroscopic returns.
+template <class ST>
+__global__ __launch_bounds__(TPB) void k_error(AuxArgs<ST> a, typename ST::R* uerr, const uint8_t* flag1)
+{
+    LBM_FP_STRICT
+    using R = typename ST::R;
+    uint32_t iu;
+    Cell c;
+    double sm[2] = {0.0, 0.0};
+    if (user_cell(a.g, iu, c)) {
+        const size_t NU = (size_t)a.g.ux * a.g.uy * a.g.uz;
+        const uint8_t cls = a.cls[c.i];
+        R rho = R(0), ux = R(0), uy = R(0), uz = R(0);
+        if (cls != CL_SOLID) cell_macro(a, c, cls, rho, ux, uy, uz, (R*)nullptr);
+        if (flag1[iu]) {
+            const R ex = uerr[iu], ey = uerr[NU + iu], ez = uerr[2 * NU + iu];
+            const R add1 = (ux - ex) * (ux - ex) + (uy - ey) * (uy - ey) + (uy - ez) * (uy - ez);
+            const R add2 = ux * ux + uy * uy + uz * uz;
+            sm[0] = (double)sqrt(add1);
+            sm[1] = (double)sqrt(add2);
+        }
+        uerr[iu] = ux;
+        uerr[NU + iu] = uy;
+        uerr[2 * NU + iu] = uz;
+    }
+    __shared__ double red[TPB / 64][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double v = sm[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        sm[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wv][0] = sm[0];
+        red[wv][1] = sm[1];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double s = 0.0;
+        for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
+        a.io[(size_t)blockIdx.x * 2 + threadIdx.x] = s;
+    }
+}
+
+// Sum of nb per-block partial records (stride doubles apart, ncomp <= 8
+// components each) into out[0 .. ncomp), by one workgroup in a fixed order:
+// thread t adds records t, t + 256, ... in turn, then a tree over the
+// threads.  Deterministic for a given nb.
+__global__ __launch_bounds__(TPB) void k_reduce_partials(const double* parts, int nb, int stride, int ncomp,
+                                                         double* out)
+{
+    __shared__ double red[TPB][8];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = (int)threadIdx.x; b < nb; b += TPB)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < ncomp) acc[k] += parts[(size_t)b * stride + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) red[threadIdx.x][k] += red[threadIdx.x + h][k];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < ncomp) out[threadIdx.x] = red[0][threadIdx.x];
 }
 
 // Per-block FP64 sums of mass and momentum over non-solid cells.

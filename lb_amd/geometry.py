@@ -58,9 +58,11 @@ def merge(parts):
     return np.concatenate(vs), np.concatenate(fs)
 
 
-def obstacle_scene(seed=20240603, domain=(512.0, 256.0, 256.0), sphere_radius=24.0, n_theta=42, n_phi=60):
+def obstacle_scene(seed=20240603, domain=(512.0, 256.0, 256.0), sphere_radius=24.0, n_theta=42, n_phi=60,
+                   ellipsoid_radii=((10, 8, 8), (28, 20, 20)), margin=4.0):
     """Returns (vertices, triangles, objects) where objects lists the analytic
-    shapes: ('sphere'|'ellipsoid', centre, radii)."""
+    shapes: ('sphere'|'ellipsoid', centre, radii).  The defaults are the
+    obstacle workload; smaller domains take smaller radii and margins."""
     rng = np.random.default_rng(seed)
     L = np.array(domain)
     objs = []
@@ -69,10 +71,10 @@ def obstacle_scene(seed=20240603, domain=(512.0, 256.0, 256.0), sphere_radius=24
     tries = 0
     while len(objs) < 4 and tries < 1000:
         tries += 1
-        r = rng.uniform([10, 8, 8], [28, 20, 20])
+        r = rng.uniform(ellipsoid_radii[0], ellipsoid_radii[1])
         cen = rng.uniform([0.35 * L[0], 0.15 * L[1], 0.15 * L[2]], [0.85 * L[0], 0.85 * L[1], 0.85 * L[2]])
-        ok = all(np.linalg.norm(cen - o[1]) > r.max() + o[2].max() + 4.0 for o in objs)
-        ok = ok and np.all(cen - r.max() > 4.0) and np.all(cen + r.max() < L - 4.0)
+        ok = all(np.linalg.norm(cen - o[1]) > r.max() + o[2].max() + margin for o in objs)
+        ok = ok and np.all(cen - r.max() > margin) and np.all(cen + r.max() < L - margin)
         if ok:
             objs.append(("ellipsoid", cen, r))
     parts = [box((0.0, 0.0, 0.0), tuple(L))]

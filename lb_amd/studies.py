@@ -83,9 +83,10 @@ def c3(n=384, steps=10000, every=1000, u0=0.05,
     return res
 
 
-def c4_geometry(workdir, res=1.0, add=0.0):
-    """Voxelized flag grid and boundary list of the obstacle configuration."""
-    v, f, objs = obstacle_scene()
+def c4_geometry(workdir, res=1.0, add=0.0, scene=None, u_in=0.04):
+    """Voxelized flag grid and boundary list of the obstacle configuration
+    (scene: (vertices, triangles, objects) of another obstacle_scene)."""
+    v, f, objs = scene if scene is not None else obstacle_scene()
     path = os.path.join(workdir, "obstacle.off")
     write_off(path, v, f)
     t0 = time.time()
@@ -102,7 +103,7 @@ def c4_geometry(workdir, res=1.0, add=0.0):
     m.wall()
     big = 1e4
     # cell centres sit at i + 0.5: the planes x = 0 and x = nx - 1
-    m.inlet((0.2, -big, -big), (0.8, big, big), (0.8, big, -big), -1, (-1.0, 0.0, 0.0), (0.04, 0.0, 0.0),
+    m.inlet((0.2, -big, -big), (0.8, big, big), (0.8, big, -big), -1, (-1.0, 0.0, 0.0), (u_in, 0.0, 0.0),
             abi.BC_INLET_EQUILIBRIUM)
     m.outlet((nx - 0.8, -big, -big), (nx - 0.2, big, big), (nx - 0.2, big, -big), -2, (1.0, 0.0, 0.0), 1.0)
     flags, bc = m.compile()

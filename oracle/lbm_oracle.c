@@ -43,6 +43,12 @@ static void orc_crev(uint32_t compat, int* crev)
 /* Weight class of a direction: number of non-zero components. */
 static int orc_nnz(int q) { return (ORC_C[q][0] != 0) + (ORC_C[q][1] != 0) + (ORC_C[q][2] != 0); }
 
+/* Deliberate mistakes (ORC_PLANT_* in lbm_oracle.h), set only by tests that
+ * show a pin rejects them; 0 in every other use. */
+static int g_plant = 0;
+
+void orc_plant(int bits) { g_plant = bits; }
+
 /* ---------------- double instantiation ---------------- */
 #define REAL double
 #define FN(x) x##_d

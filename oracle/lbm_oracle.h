@@ -48,6 +48,19 @@ void orc_destroy(orc* o);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* Deliberate mistakes, for tests that show that a pin fails on them (the
+ * oracle is otherwise always built and run with none):
+ *   NO_A          the 2017 operator's fourth-order coefficient A = 0
+ *   OUTLET_SWAP   the interpolated outlet weights c_s and 1 - c_s exchanged
+ *   INLET_NU3     the corrected equilibrium inlet uses nu/3 instead of 3 nu
+ *   BACK_221      the back-transform's 221 moment with 2 S110 S111 for 4 S110 S111
+ *   INLET_SIGN    the corrected equilibrium inlet's +x branch with the sign flipped
+ *   A_NEG         the 2017 operator's A with its sign flipped
+ *   NO_B          the 2017 operator's fourth-order coefficient B = 0 */
+enum { ORC_PLANT_NO_A = 1, ORC_PLANT_OUTLET_SWAP = 2, ORC_PLANT_INLET_NU3 = 4, ORC_PLANT_BACK_221 = 8,
+       ORC_PLANT_INLET_SIGN = 16, ORC_PLANT_A_NEG = 32, ORC_PLANT_NO_B = 64 };
+void orc_plant(int bits);
+
 /* Single-cell operators, for unit tests against closed forms.
  * f and out hold 27 values in reference direction order.  flag is the
  * cell's flag-grid value; only LBM_COLLISION_COMBINED reads it (1 selects the

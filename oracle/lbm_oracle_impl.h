@@ -93,7 +93,7 @@ enum {
  *    moments is k_000 at equilibrium, so only k_000 leaves the rest state
  *    fixed).
  * f/out in reference direction order. */
-static void FN(cumulant)(int variant, int lim210, int trace17, const FN(ck) * K_, const REAL* f, REAL rho, REAL ux,
+static void FN(cumulant)(int variant, int lim210, int trace17, int eq44, const FN(ck) * K_, const REAL* f, REAL rho, REAL ux,
                          REAL uy, REAL uz, REAL omg, REAL* out)
 {
     const REAL c13 = K_->c13;
@@ -219,10 +219,13 @@ static void FN(cumulant)(int variant, int lim210, int trace17, const FN(ck) * K_
                              omega2 * omega2 * (omega1 * (3.f * omega1 - 10.f) * (15.f * omega1 - 28.f) - 48.f));
         const REAL omega111 = omega5 + (1.f - omega5) * ORC_FABS(C_111) / (rho * lambda5 + ORC_FABS(C_111));
         const REAL omega6 = 1.f, omega7 = 1.f, omega8 = 1.f, omega9 = 1.f, omega10 = 1.f;
-        const REAL A = (4.f * omega1 * omega1 + 2.f * omega1 * omega2 * (omega1 - 6.f) +
+        const REAL A = (g_plant & ORC_PLANT_NO_A) ? (REAL)0 :
+                       ((g_plant & ORC_PLANT_A_NEG) ? -1 : 1) *
+                       (4.f * omega1 * omega1 + 2.f * omega1 * omega2 * (omega1 - 6.f) +
                         omega2 * omega2 * (omega1 * (10.f - 3.f * omega1) - 4.f)) /
                        (omega1 - omega2) / (omega2 * (2.f + 3.f * omega1) - 8.f * omega1);
-        const REAL B = (4.f * omega1 * omega2 * (9.f * omega1 - 16.f) - 4.f * omega1 * omega1 -
+        const REAL B = (g_plant & ORC_PLANT_NO_B) ? (REAL)0 :
+                       (4.f * omega1 * omega2 * (9.f * omega1 - 16.f) - 4.f * omega1 * omega1 -
                         2.f * omega2 * omega2 * (2.f + 9.f * omega1 * (omega1 - 2.f))) /
                        3.f / (omega1 - omega2) / (omega2 * (2.f + 3.f * omega1) - 8.f * omega1);
 
@@ -271,7 +274,10 @@ static void FN(cumulant)(int variant, int lim210, int trace17, const FN(ck) * K_
         /* fourth order, Eq. G2017 43-48 (:349-365) */
         const REAL Eq43 = K_->c23 * (1.f / omega1 - 0.5f) * omega6 * A * rho * (Dxu - 2.f * Dyv + Dzw) +
                           (1.f - omega6) * (C_220 - 2.f * C_202 + C_022);
-        const REAL Eq44 = K_->c23 * (1.f / omega1 - 0.5f) * omega6 * A * rho * (Dxu + Dyv - 2.f * Dzw) +
+        /* Eq. 44 as the reference writes it (LBM_COMPAT_EQ44), or with the strain
+         * combination that Eq. 43's axis convention gives (-2 Dxu + Dyv + Dzw) */
+        const REAL Eq44 = K_->c23 * (1.f / omega1 - 0.5f) * omega6 * A * rho *
+                              (eq44 ? (Dxu + Dyv - 2.f * Dzw) : (Dyv + Dzw - 2.f * Dxu)) +
                           (1.f - omega6) * (C_220 + C_202 - 2.f * C_022);
         const REAL Eq45 = -K_->c43 * (1.f / omega1 - 0.5f) * omega7 * A * rho * (Dxu + Dyv + Dzw) +
                           (1.f - omega7) * (C_220 + C_202 + C_022);
@@ -310,7 +316,8 @@ static void FN(cumulant)(int variant, int lim210, int trace17, const FN(ck) * K_
                          2.f * (S(1,1,0) * S(0,1,2) + S(1,0,1) * S(0,2,1))) / rho;
     S(2,1,2) = Cs_212 + (S(0,0,2) * S(2,1,0) + S(2,0,0) * S(0,1,2) + 4.f * S(1,0,1) * S(1,1,1) +
                          2.f * (S(0,1,1) * S(2,0,1) + S(1,1,0) * S(1,0,2))) / rho;
-    S(2,2,1) = Cs_221 + (S(2,0,0) * S(0,2,1) + S(0,2,0) * S(2,0,1) + 4.f * S(1,1,0) * S(1,1,1) +
+    S(2,2,1) = Cs_221 + (S(2,0,0) * S(0,2,1) + S(0,2,0) * S(2,0,1) +
+                         ((g_plant & ORC_PLANT_BACK_221) ? 2.f : 4.f) * S(1,1,0) * S(1,1,1) +
                          2.f * (S(1,0,1) * S(1,2,0) + S(0,1,1) * S(2,1,0))) / rho;
     S(2,2,2) = Cs_222 +
                (4.f * S(1,1,1) * S(1,1,1) + S(2,0,0) * S(0,2,2) + S(0,2,0) * S(2,0,2) + S(0,0,2) * S(2,2,0) +
@@ -421,14 +428,15 @@ static void FN(collide_one)(int collision, uint32_t compat, const REAL* w, const
                             const REAL* f, REAL rho, REAL ux, REAL uy, REAL uz, REAL omg, REAL* o)
 {
     const int lim = (compat & LBM_COMPAT_LIM210) != 0, tr = (compat & LBM_COMPAT_TRACE17) != 0;
+    const int e44 = (compat & LBM_COMPAT_EQ44) != 0;
     if (collision == LBM_COLLISION_SRT)
         FN(srt)(w, f, rho, ux, uy, uz, omg, o);
     else if (collision == LBM_COLLISION_CUM2015 || (collision == LBM_COLLISION_COMBINED && flag == 1))
-        FN(cumulant)(FN(V2015), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+        FN(cumulant)(FN(V2015), lim, tr, e44, ck, f, rho, ux, uy, uz, omg, o);
     else if (collision == LBM_COLLISION_CUM2017)
-        FN(cumulant)(FN(V2017), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+        FN(cumulant)(FN(V2017), lim, tr, e44, ck, f, rho, ux, uy, uz, omg, o);
     else
-        FN(cumulant)(FN(V2017_COMB), lim, tr, ck, f, rho, ux, uy, uz, omg, o);
+        FN(cumulant)(FN(V2017_COMB), lim, tr, e44, ck, f, rho, ux, uy, uz, omg, o);
 }
 
 /* Collision*::collision (Cum2015.h:521-536, Cum2017.h:575-590, Combined.h:1066-1086,
@@ -514,6 +522,12 @@ static void FN(op_inlet)(FN(state) * s)
                 }
             }
         } else if (b->kind == LBM_BC_INLET_EQUILIBRIUM) {
+            /* Reference (LBM_COMPAT_INLETRHO): rho_b = rho_1 + nu/3 (dux1 - duxb) on a +x
+             * normal (:80), rho_1 + nu/3 (duxb - dux1) on a -x normal (:95).  Corrected:
+             * both branches rho_1 + 3 nu (duxb - dux1), i.e. rho_b - rho_1 =
+             * 3 nu (x_b - x_1) d2u/dx2 from d(rho/3)/dx = nu d2u/dx2. */
+            const int ref = (s->compat & LBM_COMPAT_INLETRHO) != 0;
+            const REAL cnu = (g_plant & ORC_PLANT_INLET_NU3) ? s->nu / 3 : 3 * s->nu;
             const REAL uxb = b->ux, uyb = b->uy, uzb = b->uz;
             long i1, i2;
             if (b->nx == 1.0f && b->ny == 0.0f && b->nz == 0.0f) {
@@ -521,7 +535,9 @@ static void FN(op_inlet)(FN(state) * s)
                     REAL ux1 = s->u[3 * i1], ux2 = s->u[3 * i2];
                     REAL duxb = (uxb - ux1);
                     REAL dux1 = (ux1 - ux2);
-                    REAL rhob = s->rho[i1] + s->nu / 3 * (dux1 - duxb);
+                    REAL rhob = ref ? s->rho[i1] + s->nu / 3 * (dux1 - duxb)
+                                    : (g_plant & ORC_PLANT_INLET_SIGN) ? s->rho[i1] + cnu * (dux1 - duxb)
+                                                                       : s->rho[i1] + cnu * (duxb - dux1);
                     for (int v = 0; v < 27; ++v) DF(s, v, i) = FN(feq)(s->w, v, uxb, uyb, uzb, rhob);
                 }
             }
@@ -530,7 +546,7 @@ static void FN(op_inlet)(FN(state) * s)
                     REAL ux1 = s->u[3 * i1], ux2 = s->u[3 * i2];
                     REAL duxb = (ux1 - uxb);
                     REAL dux1 = (ux2 - ux1);
-                    REAL rhob = s->rho[i1] + s->nu / 3 * (duxb - dux1);
+                    REAL rhob = ref ? s->rho[i1] + s->nu / 3 * (duxb - dux1) : s->rho[i1] + cnu * (duxb - dux1);
                     for (int v = 0; v < 27; ++v) DF(s, v, i) = FN(feq)(s->w, v, uxb, uyb, uzb, rhob);
                 }
             }
@@ -635,8 +651,12 @@ static void FN(op_outlet)(FN(state) * s)
         long iu;
         if (!FN(nb)(s, b->x + (xp ? -1 : 1), b->y, b->z, &iu)) continue;
         for (int vel = 0; vel < 27; ++vel) {
-            if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] < 0)
-                DF(s, vel, i) = s->cs * DP(s, vel, iu) + (1 - s->cs) * DP(s, vel, i);
+            if (b->nx * ORC_C[vel][0] + b->ny * ORC_C[vel][1] + b->nz * ORC_C[vel][2] < 0) {
+                if (g_plant & ORC_PLANT_OUTLET_SWAP)
+                    DF(s, vel, i) = (1 - s->cs) * DP(s, vel, iu) + s->cs * DP(s, vel, i);
+                else
+                    DF(s, vel, i) = s->cs * DP(s, vel, iu) + (1 - s->cs) * DP(s, vel, i);
+            }
         }
         REAL d[27];
         FN(load)(s, s->df, i, d);

@@ -212,3 +212,41 @@ def test_refresh_holds_the_network_coefficient(gpu):
     _, _, c_new = lat.closure_features()
     assert not np.array_equal(c_new, c_fresh)
     assert np.all(np.isfinite(lat.macroscopic()[1]))
+
+
+@pytest.mark.parametrize("inlet_kind", ["moving_wall", "zou_he"])
+@pytest.mark.parametrize("refresh", [1, 10])
+def test_fallback_on_a_lattice_with_boundaries_equals_smagorinsky(gpu, inlet_kind, refresh):
+    """The flagged instantiations of the network step and of the held-
+    coefficient step (walls, a moving-wall or Zou-He inlet, an interpolated
+    outlet, ghost layers): with an empty envelope every cell falls back, and
+    the run equals the Smagorinsky run on the same lattice to FP32 rounding."""
+    from helpers import bc_nodes
+    from oracle import mesher_oracle as mesh
+
+    kind = mesh.BC_INLET_MOVING_WALL if inlet_kind == "moving_wall" else mesh.BC_INLET_ZOU_HE
+    flags, bcd = mesh.channel(32, 16, 12, inlet_kind=kind)
+    bc = bc_nodes(bcd)
+    assert (flags == 0).any() and (flags == -1).any() and (flags == -2).any() and (flags == -3).any()
+    p, _, _, _ = _problem(16, 18)
+    norm = _open_norm()
+    norm[2], norm[3] = 1.0, -1.0
+    kw = dict(collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F32, arith=abi.ARITH_FAST, compat=0,
+              tau=0.53, nu=0.01, c_les=C_LES)
+    a = Lattice(make_desc(flags, bc, closure=abi.CLOSURE_MLP, **kw))
+    a.set_closure(p, norm)
+    a.set_closure_refresh(refresh)
+    b = Lattice(make_desc(flags, bc, closure=abi.CLOSURE_SMAGORINSKY, **kw))
+    rng = np.random.default_rng(19)
+    solid = flags == 0
+    ux = np.where(solid, 0.0, 0.04 + 0.004 * rng.standard_normal(flags.shape))
+    uy = np.where(solid, 0.0, 0.004 * rng.standard_normal(flags.shape))
+    for L in (a, b):
+        L.init_equilibrium(None, ux, uy, None)
+        L.step(30)
+    fa, fb = a.populations(), b.populations()
+    e = rel_l2(fa, fb)
+    print(f"[fallback with boundaries, {inlet_kind}, refresh {refresh}] relL2 {e:.3e}", flush=True)
+    assert np.all(np.isfinite(fa)) and e <= 1e-6
+    _, env, coef = a.closure_features()
+    assert not env.any()

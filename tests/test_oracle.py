@@ -385,9 +385,9 @@ def test_outlet_resets_density_and_inlet_holds_equilibrium():
     """After every step the outlet nodes hold total mass 1 exactly (exact
     weights: f - f_eq(u, rho) + f_eq(u, 1) with sum f_eq(u, r) = r,
     OutletDensityInterpolated.h:244-246), and every inlet node holds
-    f_eq(u_b, rho_b) with rho_b = rho(x+1) - (nu/3)(u_b - 2 u(x+1) + u(x+2))
-    from the neighbours' moments entering the step (InletVelocityEquilibrium.h:80-92,
-    written here as a second difference)."""
+    f_eq(u_b, rho_b) with rho_b = rho(x+1) - 3 nu (u_b - 2 u(x+1) + u(x+2))
+    from the neighbours' moments entering the step (InletVelocityEquilibrium.h:80-92
+    with the corrected coefficient, written here as a second difference)."""
     nu = 0.02
     flags, bcd, bc = _channel(16, 8, 6, u_in=0.05)
     o = OracleLattice(make_desc(flags, bc, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_SMAGORINSKY,
@@ -408,7 +408,7 @@ def test_outlet_resets_density_and_inlet_holds_equilibrium():
         for d in inlets:
             x, y, z = d["x"], d["y"], d["z"]
             ub = np.float64(np.float32(d["velocity"][0]))
-            rb = rho0[z, y, x + 1] - nu / 3 * (ub - 2 * ux0[z, y, x + 1] + ux0[z, y, x + 2])
+            rb = rho0[z, y, x + 1] - 3 * nu * (ub - 2 * ux0[z, y, x + 1] + ux0[z, y, x + 2])
             got = f[:, z, y, x]
             assert abs(got.sum() - rb) < 1e-13
             assert np.abs(got - feq(rb, (ub, 0.0, 0.0))).max() < 1e-15

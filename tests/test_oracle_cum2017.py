@@ -19,10 +19,9 @@ brute-force definition; here the relaxation block is pinned:
 - the Combined operator collides flag-1 cells with the 2015 operator and other
   fluid cells with the 2017 one;
 - physics: the optimised omega_3..5 and the B term of Geier 2017 make the
-  viscosity of a shear wave converge faster than the 2015 operator's, whose
-  rates are all 1 (an error in those rational functions loses that).
-The coefficient A(omega_1) enters only through diagonal strain and is pinned
-by restatement alone.
+  viscosity error of a shear wave less than half the 2015 operator's, whose
+  rates are all 1 (both second order).  The coefficients A and B are pinned
+  further in test_oracle_pins.py.
 """
 import numpy as np
 import pytest
@@ -220,39 +219,22 @@ def test_combined_lattice_with_interior_only_is_the_2015_operator():
     assert np.array_equal(out[0], out[1])
 
 
-def _shear_nu_error(coll, L, tau):
-    """Relative error of the viscosity measured from a decaying shear wave
-    u_y(x) = U sin(2 pi x / L) on a periodic L x 1 x 1 lattice."""
-    shape = (1, 1, L)
-    x = np.arange(L)
-    mode = np.sin(2 * np.pi * x / L)
-    d = make_desc(np.ones(shape, np.int32), [], collision=coll, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
-                  compat=0, periodic=(1, 1, 1), tau=tau)
-    o = OracleLattice(d)
-    o.init_equilibrium(None, None, np.broadcast_to(1e-4 * mode, shape).copy(), None)
-    nu = (tau - 0.5) / 3
-    k2 = (2 * np.pi / L) ** 2
-    steps = int(0.3 / (nu * k2))
-    o.step(20)
-
-    def amp():
-        return 2 * np.mean(o.macroscopic()[2][0, 0] * mode)
-
-    a0 = amp()
-    o.step(steps)
-    a1 = amp()
-    return (-np.log(a1 / a0) / (k2 * steps)) / nu - 1
-
-
-def test_shear_wave_viscosity_converges_faster_than_2015():
+def test_shear_wave_viscosity_error_is_smaller_than_2015():
+    """Viscosity error of the axis-aligned shear wave u_y(x) (k || x), from a
+    fit of the log amplitude after the initial layer: both operators are
+    second order (error ratio 3.5-4.5 per halving of the wavelength), and the
+    2017 operator's optimised omega_3..5 and B make its error less than half
+    the 2015 operator's.  (A two-point estimate taken shortly after the
+    equilibrium start mixes in the initial layer and can suggest a faster
+    order; the fit does not.)"""
+    from test_oracle_pins import _wave_nu_error
     tau = 0.526   # omega_1 = 1.90: omega_3..5 = 0.41, 0.047, 0.16, all in (0, 2)
-    e15 = [abs(_shear_nu_error(C15, L, tau)) for L in (16, 32)]
-    e17 = [abs(_shear_nu_error(C17, L, tau)) for L in (16, 32)]
-    print(f"shear-wave nu error L=16,32: 2015 {e15[0]:.3e} {e15[1]:.3e}; 2017 {e17[0]:.3e} {e17[1]:.3e}")
-    # 2015: second order (error ratio about 4 per halving); 2017: faster
-    assert 3.0 < e15[0] / e15[1] < 5.0
-    assert e17[0] / e17[1] > 6.0
-    assert e17[1] < 0.1 * e15[1]
+    e15 = [_wave_nu_error(L, tau, False, coll=C15) for L in (32, 64)]
+    e17 = [_wave_nu_error(L, tau, False, coll=C17) for L in (32, 64)]
+    print(f"shear-wave nu error L=32,64: 2015 {e15[0]:.3e} {e15[1]:.3e}; 2017 {e17[0]:.3e} {e17[1]:.3e}")
+    for e in (e15, e17):
+        assert 3.5 < e[0] / e[1] < 4.5
+    assert abs(e17[0]) < 0.5 * abs(e15[0]) and abs(e17[1]) < 0.5 * abs(e15[1])
 
 
 def test_product_collision_arithmetic_is_bitwise_the_oracle_2017():
