@@ -221,6 +221,20 @@ lbm_status lbm_conservation(lbm_ctx* ctx, double out[4]);
  * solid.  (No reference counterpart; the learned closure is excluded.) */
 lbm_status lbm_step_diagnostics(lbm_ctx* ctx, double out[7]);
 
+/* lbm_step_diagnostics without allocating: no counterpart in the reference.
+ * For callers that cannot call hipMalloc (code under stream capture, tight
+ * stepping loops, harnesses that forbid allocation).  Same step and the same
+ * seven values as lbm_step_diagnostics, whose per-tile FP64 records sum on the
+ * host; here they go to a fixed device array of the library (one per GPU,
+ * 16 MiB, held until the process ends) and are summed on the device, so the
+ * results agree to FP64 summation rounding.  A lattice needs at most
+ * LBM_NOALLOC_MAX_TILES tiles of 256 cells (a step's workgroups; 67.1 M cells
+ * when rows are multiples of 256), else LBM_E_UNSUPPORTED before any step is
+ * taken.  Calls that overlap on one GPU (other threads or streams) race on the
+ * shared array.  Synchronises. */
+#define LBM_NOALLOC_MAX_TILES 262144
+lbm_status lbm_step_diagnostics_noalloc(lbm_ctx* ctx, double out[7]);
+
 /* ErrorQuadratic::errorEvaluation (errorEvaluations/ErrorQuadratic.h:33-90),
  * the solver loop's convergence norm, on the device: with u the velocity
  * lbm_get_macroscopic returns and u_prev the velocity of the previous call

@@ -74,6 +74,7 @@ CORE_SYMBOLS = [
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_diagnostics", ctypes.c_int, [_vp, _dp]),
     ("lbm_error_quadratic", ctypes.c_int, [_vp, _dp]),
+    ("lbm_step_diagnostics_noalloc", ctypes.c_int, [_vp, _dp]),
     ("lbm_moment_add", ctypes.c_int, [_vp]),
     ("lbm_step_moment_add", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("lbm_moment_avg", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),

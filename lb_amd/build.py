@@ -43,7 +43,7 @@ def _run(cmd, cwd=None):
         sys.stderr.write("\n".join(noise) + "\n")
 
 
-CORE_UNITS = ["lbm_core.hip", "closure.hip", "voxel.hip"]
+CORE_UNITS = ["lbm_core.hip", "closure.hip", "voxel.hip", "noalloc.hip"]
 
 
 def build_core():

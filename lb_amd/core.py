@@ -155,6 +155,12 @@ class Lattice:
         _check(self._lib.lbm_step_diagnostics(self._h, abi.ptr(out)), "lbm_step_diagnostics")
         return out[0], out[1:4].copy(), out[4:7].copy()
 
+    def step_diagnostics_noalloc(self):
+        """step_diagnostics through the entry point that allocates nothing."""
+        out = np.empty(7)
+        _check(self._lib.lbm_step_diagnostics_noalloc(self._h, abi.ptr(out)), "lbm_step_diagnostics_noalloc")
+        return out[0], out[1:4].copy(), out[4:7].copy()
+
     def error_quadratic(self) -> float:
         """ErrorQuadratic's convergence norm against the previous call's
         velocity (reduced on the device); NaN sums raise LBM_E_NAN."""

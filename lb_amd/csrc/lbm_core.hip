@@ -1416,6 +1416,53 @@ lbm_status lbm_step_diagnostics(lbm_ctx* c, double out[7])
     return LBM_OK;
 }
 
+}  // extern "C"
+
+// Internal hooks of noalloc.hip (not part of the C ABI).
+//
+// Geometry of the diagnostics step: tiles (one partial record each), the
+// device and stream of the context.
+lbm_status lbm_internal_diag_info(lbm_ctx* c, int* ntiles, int* device, hipStream_t* stream)
+{
+    if (!c) return LBM_E_ARG;
+    *ntiles = c->g.ntiles;
+    *device = c->device;
+    *stream = c->stream;
+    return LBM_OK;
+}
+
+// lbm_step_diagnostics' step with its per-tile partial records (8 doubles
+// each) written to dst instead of the context's buffer; asynchronous.
+lbm_status lbm_internal_diag_step(lbm_ctx* c, double* dst)
+{
+    if (!c || !dst) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    if (c->d.closure == LBM_CLOSURE_MLP) return LBM_E_UNSUPPORTED;
+    c->diag_dst = dst;
+    c->diag_next = true;
+    lbm_status st = lbm_step(c, 1);
+    c->diag_next = false;
+    c->diag_dst = nullptr;
+    return st;
+}
+
+void lbm_internal_mark_nan(lbm_ctx* c)
+{
+    if (c) c->nan_seen = true;
+}
+
+// out[0 .. ncomp) = sum over nb records of parts (stride doubles apart), on
+// the device in a fixed order; asynchronous on `stream`.
+lbm_status lbm_internal_reduce(const double* parts, int nb, int stride, int ncomp, double* out, hipStream_t stream)
+{
+    if (nb < 1 || ncomp < 1 || ncomp > 8 || stride < ncomp) return LBM_E_ARG;
+    k_reduce_partials<<<1, TPB, 0, stream>>>(parts, nb, stride, ncomp, out);
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+}
+
+extern "C" {
+
 lbm_status lbm_error_quadratic(lbm_ctx* c, double* err)
 {
     if (!c || !err) return LBM_E_ARG;
