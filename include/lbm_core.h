@@ -146,6 +146,13 @@ typedef struct lbm_desc {
      * at the high end of that axis.  Needs an axis-aligned normal. */
     int32_t outlet_sponge;     /* 1 = on */
     float sponge_low, sponge_high;   /* LBMConstants::omegaDumpingLow / High */
+    /* Wall policy.  0: BounceBackWallHalf (Wall/BounceBackWallHalf.h:29-52), half-way
+     * bounce-back on the nodes of the wall list (LBM_BC_WALL entries; the mesher
+     * lists cells with flag > 2).  1: BounceBackWallHalfMesh
+     * (Wall/BounceBackWallHalfMesh.h:38-50), on every cell whose flag is exactly 2
+     * (one solid neighbour), whatever the wall list holds.  Other fluid cells keep
+     * the populations their solid neighbours would send (StreamingAB.h:58-65). */
+    int32_t wall_mesh;
 } lbm_desc;
 
 /* Create a context on the current HIP device.  Validates the flag grid and the
@@ -184,6 +191,27 @@ lbm_status lbm_get_macroscopic(lbm_ctx* ctx, double* rho, double* ux, double* uy
 /* Current (post-stream, post-boundary) populations, [q][z][y][x], FP64;
  * 0 on solid cells.  Synchronises. */
 lbm_status lbm_get_populations(lbm_ctx* ctx, double* f);
+
+/* Population q of every cell (as lbm_get_populations, one plane): what
+ * outputerVTK::distributionFunctionVTK writes (postprocesors/outputerVTK.h:
+ * 660-740).  Synchronises. */
+lbm_status lbm_get_population(lbm_ctx* ctx, int32_t q, double* f);
+
+/* Output snapshots that do not stall the stepping.  The reference writes VTK
+ * output at its plot cadence by copying rho and u to the host and waiting
+ * (SolverTurbulentLES.h:243-249, outputerVTK.h:59-120).  lbm_snapshot_begin
+ * queues, behind the steps already queued, a kernel that forms rho, u (the
+ * values of lbm_get_macroscopic) and omega (lbm_get_omega) of that state into
+ * a device buffer, and a copy of it into pinned host memory on a second
+ * stream; it returns at once, and the steps queued after it run while the copy
+ * goes on.  lbm_snapshot_wait waits for that copy only and hands out the
+ * planes (any pointer may be NULL) and the step count of the state.  One
+ * snapshot at a time: begin while another is pending is LBM_E_STATE.  The
+ * first begin allocates 5 values per cell on the device and in pinned host
+ * memory. */
+lbm_status lbm_snapshot_begin(lbm_ctx* ctx);
+lbm_status lbm_snapshot_wait(lbm_ctx* ctx, double* rho, double* ux, double* uy, double* uz, double* omega,
+                             int64_t* step);
 
 /* rho, ux, uy, uz of one cell (as lbm_get_macroscopic) without copying the
  * whole field: the reference's probeWrite reads four values

@@ -55,6 +55,24 @@ int lbmh_mesher_wall(lbmh_mesher* m);
  * Returns 1 where the reference throws (all velocity components equal). */
 int lbmh_mesher_inlet(lbmh_mesher* m, const double p1[3], const double p2[3], const double p3[3], int32_t id,
                       const float normal[3], const float velocity[3], int32_t kind);
+/* meshingBoundaryConditionInletParaboloidCircle (:111-178) and
+ * ...ParaboloidRectangle (:180-274): the cuboid's fluid cells become inlet
+ * nodes (kind = LBM_BC_INLET_*), then every inlet entry whose centre lies in
+ * the cuboid gets velocity -v n with, for the circle, v = 3 u_mean (1 - (r/R)^2)
+ * within radius R of `center` (0 outside; its mean over the disc is 1.5 u_mean),
+ * and for the rectangle v = 2 u_mean prod_i d_i (L_i - d_i) / prod_i (L_i/2)^2
+ * over the axes across the normal (d_i from the rectangle's low corner, L_i =
+ * dims[i]; peak 2 u_mean).  The rectangle version sets dims[] along the normal
+ * to 1, as the reference's reference parameters are.  The reference's two
+ * functions start from the uniform inlet with velocity (0, 0, 0), which that
+ * function rejects (:52-54), so as written they throw; here the cells are
+ * meshed without that check.  Returns 0, or 1 for a NULL argument. */
+int lbmh_mesher_inlet_paraboloid_circle(lbmh_mesher* m, const double p1[3], const double p2[3], const double p3[3],
+                                        int32_t id, const float normal[3], const double center[3], float radius,
+                                        float mean_velocity, int32_t kind);
+int lbmh_mesher_inlet_paraboloid_rectangle(lbmh_mesher* m, const double p1[3], const double p2[3],
+                                           const double p3[3], int32_t id, const float normal[3],
+                                           const double center[3], float dims[3], float mean_velocity, int32_t kind);
 /* meshingBoundaryConditionOutlet (:276-325) */
 int lbmh_mesher_outlet(lbmh_mesher* m, const double p1[3], const double p2[3], const double p3[3], int32_t id,
                        const float normal[3], float density);
@@ -85,6 +103,12 @@ int lbmh_case_set_grid(lbmh_case* c, const int32_t dims[3], const int32_t* grid0
 int lbmh_case_mesh_wall(lbmh_case* c);
 int lbmh_case_inlet(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3], int32_t id,
                     const float normal[3], const float velocity[3], int32_t kind);
+int lbmh_case_inlet_paraboloid_circle(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3],
+                                      int32_t id, const float normal[3], const double center[3], float radius,
+                                      float mean_velocity, int32_t kind);
+int lbmh_case_inlet_paraboloid_rectangle(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3],
+                                         int32_t id, const float normal[3], const double center[3], float dims[3],
+                                         float mean_velocity, int32_t kind);
 int lbmh_case_outlet(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3], int32_t id,
                      const float normal[3], float density);
 int lbmh_case_periodic(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3], int32_t id,
@@ -109,6 +133,16 @@ lbm_ctx* lbmh_case_lattice(lbmh_case* c);
  * velocity. */
 int lbmh_read_vtk_fields(const char* path, const int32_t dims[3], double* rho, double* ux, double* uy, double* uz);
 int lbmh_case_set_init_file(lbmh_case* c, const char* name);
+
+/* outputerVTK writers (postprocesors/outputerVTK.h:59-822) on the case's
+ * current state, into the results directory: which = "variables" (physical
+ * units, file variables.<index>.vtk), "variablesLattice" (variablesLattice<index>.vtk),
+ * "omega", "tau", "distributionFunction" (population <index>,
+ * distributionFunction<index>.vtk), "variablesTimeAvg" or
+ * "variablesTimeAvgLattice" (the averages of the last run; both write
+ * variablesTimeAvg.vtk, as the reference's do).  Returns 0, 1 (unknown name or
+ * no averages) or 2 (read or write failed). */
+int lbmh_case_write_vtk(lbmh_case* c, const char* which, int32_t index);
 
 /* Error norms recorded by the loop (ErrorQuadratic), oldest first. */
 int lbmh_case_errors(lbmh_case* c, double* out, int32_t* n);

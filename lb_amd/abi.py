@@ -52,7 +52,8 @@ class Desc(ctypes.Structure):
                 ("tau", ctypes.c_double), ("nu", ctypes.c_double), ("c_les", ctypes.c_double),
                 ("rho0", ctypes.c_double),
                 ("flags", ctypes.POINTER(ctypes.c_int32)), ("bc", ctypes.POINTER(BcNode)),
-                ("outlet_sponge", ctypes.c_int32), ("sponge_low", ctypes.c_float), ("sponge_high", ctypes.c_float)]
+                ("outlet_sponge", ctypes.c_int32), ("sponge_low", ctypes.c_float), ("sponge_high", ctypes.c_float),
+                ("wall_mesh", ctypes.c_int32)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -74,6 +75,9 @@ CORE_SYMBOLS = [
     ("lbm_conservation", ctypes.c_int, [_vp, _dp]),
     ("lbm_step_diagnostics", ctypes.c_int, [_vp, _dp]),
     ("lbm_error_quadratic", ctypes.c_int, [_vp, _dp]),
+    ("lbm_get_population", ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    ("lbm_snapshot_begin", ctypes.c_int, [_vp]),
+    ("lbm_snapshot_wait", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     ("lbm_step_diagnostics_noalloc", ctypes.c_int, [_vp, _dp]),
     ("lbm_moment_add", ctypes.c_int, [_vp]),
     ("lbm_step_moment_add", ctypes.c_int, [_vp, ctypes.c_int32]),

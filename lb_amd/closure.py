@@ -268,9 +268,9 @@ def dissipation(t, K):
 def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_seed=20240601, nu_fine=4e-3,
                    train_steps=3000, train_snaps=(1000, 2000, 3000), eval_steps=4000, eval_every=100, batch=32768,
                    adam_steps=3000, lr=1e-3, c_les=0.17, seed=7, refresh=10, extra_envelopes=((0.0005, 0.9995),),
-                   out_json=None):
+                   out_json=None, f_eval=2):
     """The learned-closure experiment: data from a fine run filtered by 4,
-    training, then LES at fine_n/2 against the fine run filtered by 2."""
+    training, then LES at fine_n/f_eval against the fine run filtered by f_eval."""
     res = {"fine_n": fine_n, "nu_fine": nu_fine, "c_les": c_les, "batch": batch}
     # 1. training data
     f_train = 4
@@ -304,8 +304,8 @@ def run_experiment(fine_n=512, train_k0=8, train_seed=20240602, eval_k0=4, eval_
     params = m.params()
     _log(f"trained {adam_steps} steps in {res['train_seconds']:.1f} s: loss {res['loss_first']:.3e} -> "
          f"{res['loss_last']:.3e} (Smagorinsky {res['loss_smagorinsky']:.3e})")
-    # 3. evaluation: fine run filtered by 2 vs LES on the coarse grid
-    f_eval = 2
+    # 3. evaluation: fine run filtered by f_eval vs LES on the coarse grid
+    res["f_eval"] = f_eval
     nu_e = nu_fine * f_eval / (f_eval * f_eval)
     fine_t, fine_K, fine_final, filt0 = [], [], None, None
     for step, filt in fine_run(fine_n, eval_k0, eval_seed, nu_fine, eval_steps, eval_every * f_eval, f_eval):

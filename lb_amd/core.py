@@ -23,7 +23,7 @@ def _check(status: int, where: str):
 
 def make_desc(flags: np.ndarray, bc=None, *, collision=abi.COLLISION_CUM2015, closure=abi.CLOSURE_NONE,
               storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL, compat=abi.COMPAT_REFERENCE,
-              periodic=(0, 0, 0), tau=0.6, nu=0.0, c_les=0.0, rho0=1.0, sponge=None):
+              periodic=(0, 0, 0), tau=0.6, nu=0.0, c_les=0.0, rho0=1.0, sponge=None, wall_mesh=False):
     """Build an lbm_desc.  flags: int32 array shaped (nz, ny, nx) (x fastest in memory)."""
     flags = np.ascontiguousarray(flags, dtype=np.int32)
     nz, ny, nx = flags.shape
@@ -32,6 +32,7 @@ def make_desc(flags: np.ndarray, bc=None, *, collision=abi.COLLISION_CUM2015, cl
     d.periodic[:] = [int(bool(p)) for p in periodic]
     d.collision, d.closure, d.storage, d.arith, d.compat = collision, closure, storage, arith, compat
     d.tau, d.nu, d.c_les, d.rho0 = float(tau), float(nu), float(c_les), float(rho0)
+    d.wall_mesh = int(bool(wall_mesh))   # BounceBackWallHalfMesh: flag-2 cells instead of the wall list
     if sponge is not None:   # (omegaDumpingLow, omegaDumpingHigh) of the outlet omega damping
         d.outlet_sponge, d.sponge_low, d.sponge_high = 1, float(sponge[0]), float(sponge[1])
     d.flags = flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
@@ -160,6 +161,24 @@ class Lattice:
         out = np.empty(7)
         _check(self._lib.lbm_step_diagnostics_noalloc(self._h, abi.ptr(out)), "lbm_step_diagnostics_noalloc")
         return out[0], out[1:4].copy(), out[4:7].copy()
+
+    def population(self, q: int):
+        """Population plane q of the current state, shaped (nz, ny, nx)."""
+        f = np.empty(self.shape)
+        _check(self._lib.lbm_get_population(self._h, int(q), abi.ptr(f)), "lbm_get_population")
+        return f
+
+    def snapshot_begin(self):
+        """Queue a snapshot of rho, u, omega behind the queued steps (returns at once)."""
+        _check(self._lib.lbm_snapshot_begin(self._h), "lbm_snapshot_begin")
+
+    def snapshot_wait(self):
+        """(step, [rho, ux, uy, uz, omega]) of the pending snapshot."""
+        out = [np.empty(self.shape) for _ in range(5)]
+        st = ctypes.c_int64()
+        _check(self._lib.lbm_snapshot_wait(self._h, *[abi.ptr(a) for a in out], ctypes.byref(st)),
+               "lbm_snapshot_wait")
+        return st.value, out
 
     def error_quadratic(self) -> float:
         """ErrorQuadratic's convergence norm against the previous call's

@@ -133,6 +133,27 @@ int lbmh_mesher_inlet(lbmh_mesher* m, const double p1[3], const double p2[3], co
                : 1;
 }
 
+int lbmh_mesher_inlet_paraboloid_circle(lbmh_mesher* m, const double p1[3], const double p2[3], const double p3[3],
+                                        int32_t id, const float normal[3], const double center[3], float radius,
+                                        float mean_velocity, int32_t kind)
+{
+    if (!m || !center) return 1;
+    m->M.meshingBoundaryConditionInletParaboloidCircle(cuboid(p1, p2, p3, id), {center[0], center[1], center[2]}, radius,
+                                                       {normal[0], normal[1], normal[2]}, mean_velocity, false, kind);
+    return 0;
+}
+
+int lbmh_mesher_inlet_paraboloid_rectangle(lbmh_mesher* m, const double p1[3], const double p2[3],
+                                           const double p3[3], int32_t id, const float normal[3],
+                                           const double center[3], float dims[3], float mean_velocity, int32_t kind)
+{
+    if (!m || !center || !dims) return 1;
+    m->M.meshingBoundaryConditionInletParaboloidRectangle(cuboid(p1, p2, p3, id), {center[0], center[1], center[2]},
+                                                          dims[0], dims[1], dims[2], {normal[0], normal[1], normal[2]},
+                                                          mean_velocity, false, kind);
+    return 0;
+}
+
 int lbmh_mesher_outlet(lbmh_mesher* m, const double p1[3], const double p2[3], const double p3[3], int32_t id,
                        const float normal[3], float density)
 {
@@ -201,7 +222,7 @@ Field field(LBMConstants& C, const std::string& n)
     F(BBminx) F(BBmaxx) F(BBminy) F(BBmaxy) F(BBminz) F(BBmaxz) F(resolution_factor) F(additional_factor)
     F(ny_fyz) F(u_guess_fyz) F(L_fyz) F(rho_fyz) F(Re) F(conversion_factor_fyz) F(tau) F(ny) F(omega) F(U_lb)
     F(Cl) F(Ct) F(Cm) F(Cu_inverse) F(Crho) F(Cpressure) F(Cu) F(CLES) F(err) F(time) F(plot_every) F(err_every)
-    F(omegaDumpingLow) F(omegaDumpingHigh) I(outlet_sponge)
+    F(omegaDumpingLow) F(omegaDumpingHigh) I(outlet_sponge) I(wall_mesh) I(async_output)
     I(dimX_int) I(dimY_int) I(dimZ_int) I(inlet_num) I(outlet_num) I(periodic_num) I(wall_num) I(plot_every_it)
     I(err_every_it) I(iterations) I(iterationsMomentAvg) I(probe_every_it) I(probe_iterations) I(Nvel)
     I(collision) I(closure) I(storage) I(arith)
@@ -303,6 +324,49 @@ int lbmh_case_inlet(lbmh_case* c, const double p1[3], const double p2[3], const 
                                                      {velocity[0], velocity[1], velocity[2]}, false, kind)
                ? 0
                : 1;
+}
+
+int lbmh_case_inlet_paraboloid_circle(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3],
+                                      int32_t id, const float normal[3], const double center[3], float radius,
+                                      float mean_velocity, int32_t kind)
+{
+    if (!c || !center) return 1;
+    c->M.meshingBoundaryConditionInletParaboloidCircle(cuboid(p1, p2, p3, id), {center[0], center[1], center[2]}, radius,
+                                                       {normal[0], normal[1], normal[2]}, mean_velocity, false, kind);
+    return 0;
+}
+
+int lbmh_case_inlet_paraboloid_rectangle(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3],
+                                         int32_t id, const float normal[3], const double center[3], float dims[3],
+                                         float mean_velocity, int32_t kind)
+{
+    if (!c || !center || !dims) return 1;
+    c->M.meshingBoundaryConditionInletParaboloidRectangle(cuboid(p1, p2, p3, id), {center[0], center[1], center[2]},
+                                                          dims[0], dims[1], dims[2], {normal[0], normal[1], normal[2]},
+                                                          mean_velocity, false, kind);
+    return 0;
+}
+
+int lbmh_case_write_vtk(lbmh_case* c, const char* which, int32_t index)
+{
+    if (!c || !which) return 1;
+    const std::string w(which);
+    bool ok;
+    if (w == "variables") ok = outputerVTK::variablesVTK(c->D, c->C, index, false);
+    else if (w == "variablesLattice") ok = outputerVTK::variablesLatticeVTK(c->D, c->C, index, false);
+    else if (w == "omega") ok = outputerVTK::omegaVTK(c->D, c->C, false);
+    else if (w == "tau") ok = outputerVTK::tauVTK(c->D, c->C, false);
+    else if (w == "distributionFunction") {
+        if (index < 0 || index >= LBM_Q) return 1;
+        ok = outputerVTK::distributionFunctionVTK(c->D, c->C, index, false);
+    } else if (w == "variablesTimeAvg" || w == "variablesTimeAvgLattice") {
+        if (c->D.rho_avg.size() != c->D.meshFluidHost.size()) return 1;
+        ok = w == "variablesTimeAvg" ? outputerVTK::variablesTimeAvgVTK(c->D, c->C, false)
+                                     : outputerVTK::variablesTimeAvgLatticeVTK(c->D, c->C, false);
+    } else {
+        return 1;
+    }
+    return ok ? 0 : 2;
 }
 
 int lbmh_case_outlet(lbmh_case* c, const double p1[3], const double p2[3], const double p3[3], int32_t id,

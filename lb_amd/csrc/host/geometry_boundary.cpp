@@ -48,6 +48,96 @@ bool geometryMesherBoundary::meshingBoundaryConditionInletUniform(const geometry
     if (velocity_in.x == velocity_in.y && velocity_in.x == velocity_in.z) return false;  // reference throws (:52-54)
     if (std::abs(normal_out.x + normal_out.y + normal_out.z) != 1)
         std::printf("Your normal is not tangent to one of the axes.\n");
+    const int num = meshInletCells(cuboid, normal_out, velocity_in, kind);
+    if (verbose) std::cout << "\nMeshed cuboid id " << cuboid.id << " as Inlet.\nBoundary vertexes number: " << num << "\n";
+    return true;
+}
+
+// The parabolic inlets start from the uniform rule with velocity (0, 0, 0)
+// (:124, :200), which the uniform rule's own check rejects (:52-54: all
+// components equal) -- as written they throw.  Here the cells are meshed
+// without that check, which is what the two functions go on to assume.
+void geometryMesherBoundary::meshingBoundaryConditionInletParaboloidCircle(
+    const geometryObjectCuboid& cuboid, const d3& inletCenter, RealType inletRadius, const Vector& normal_out,
+    RealType mean_velocity_in, bool verbose, int kind)
+{
+    if (std::abs(normal_out.x + normal_out.y + normal_out.z) != 1)
+        std::printf("Your normal is not tangent to one of the axes.\n");
+    meshInletCells(cuboid, normal_out, Vector{}, kind);
+    int num = 0;
+    for (boundaryConditionInlet& BC : boundary_vector_inlet) {
+        const d3 c = cellCentre(BC.vertex.x, BC.vertex.y, BC.vertex.z);
+        const RealType lookx = (RealType)c.x, looky = (RealType)c.y, lookz = (RealType)c.z;
+        if (!cuboid.isInside({lookx, looky, lookz})) continue;
+        // (:137-141): double differences and root, held in RealType
+        const RealType distance = (RealType)std::sqrt(std::pow(lookx - inletCenter.x, 2) +
+                                                      std::pow(looky - inletCenter.y, 2) +
+                                                      std::pow(lookz - inletCenter.z, 2));
+        if (distance <= inletRadius) {
+            // (:147): 3 u_mean (1 - (r/R)^2), the power in double
+            RealType v = (RealType)(3 * mean_velocity_in * (1.0 - std::pow(distance / inletRadius, 2)));
+            if (v < 0) v = 0;
+            BC.velocity.x = -v * normal_out.x;
+            BC.velocity.y = -v * normal_out.y;
+            BC.velocity.z = -v * normal_out.z;
+            ++num;
+        }
+    }
+    if (verbose) std::cout << "\nMeshed cuboid id " << cuboid.id << " as Inlet.\nBoundary vertexes number: " << num << "\n";
+}
+
+void geometryMesherBoundary::meshingBoundaryConditionInletParaboloidRectangle(
+    const geometryObjectCuboid& cuboid, const d3& inletCenter, RealType& inletDimX, RealType& inletDimY,
+    RealType& inletDimZ, const Vector& normal_out, RealType mean_velocity_in, bool verbose, int kind)
+{
+    if (std::abs(normal_out.x + normal_out.y + normal_out.z) != 1)
+        std::printf("Your normal is not tangent to one of the axes.\n");
+    int num = 0;
+    meshInletCells(cuboid, normal_out, Vector{}, kind);
+    for (boundaryConditionInlet& BC : boundary_vector_inlet) {
+        const d3 c = cellCentre(BC.vertex.x, BC.vertex.y, BC.vertex.z);
+        const RealType lookx = (RealType)c.x, looky = (RealType)c.y, lookz = (RealType)c.z;
+        if (!cuboid.isInside({lookx, looky, lookz})) continue;
+        // distances from the corner with the smallest coordinates (:214-216)
+        const RealType distance_x = (RealType)(lookx - (inletCenter.x - inletDimX / 2));
+        const RealType distance_y = (RealType)(looky - (inletCenter.y - inletDimY / 2));
+        const RealType distance_z = (RealType)(lookz - (inletCenter.z - inletDimZ / 2));
+        // a product of 1-D parabolas across the inlet (:225-252); the extent
+        // along the normal becomes 1
+        RealType Part_x, Part_y, Part_z;
+        if (normal_out.x == 0) {
+            Part_x = distance_x * (inletDimX - distance_x);
+        } else {
+            Part_x = 1.f;
+            inletDimX = 1.f;
+        }
+        if (normal_out.y == 0) {
+            Part_y = distance_y * (inletDimY - distance_y);
+        } else {
+            Part_y = 1.f;
+            inletDimY = 1.f;
+        }
+        if (normal_out.z == 0) {
+            Part_z = distance_z * (inletDimZ - distance_z);
+        } else {
+            Part_z = 1.f;
+            inletDimZ = 1.f;
+        }
+        const RealType inletConstant = inletDimX * inletDimY * inletDimZ * inletDimX * inletDimY * inletDimZ / 16;
+        RealType v = 2 * mean_velocity_in * Part_x * Part_y * Part_z / inletConstant;
+        if (v < 0) v = 0;
+        BC.velocity.x = -v * normal_out.x;
+        BC.velocity.y = -v * normal_out.y;
+        BC.velocity.z = -v * normal_out.z;
+        ++num;
+    }
+    if (verbose) std::cout << "\nMeshed cuboid id " << cuboid.id << " as Inlet.\nBoundary vertexes number: " << num << "\n";
+}
+
+// Cells of the cuboid with a fluid flag become inlet nodes (:58-100).
+int geometryMesherBoundary::meshInletCells(const geometryObjectCuboid& cuboid, const Vector& normal_out,
+                                           const Vector& velocity_in, int kind)
+{
     int num = 0;
     for (int k = 0; k < Constants.dimZ_int; k++)
         for (int j = 0; j < Constants.dimY_int; j++)
@@ -66,8 +156,7 @@ bool geometryMesherBoundary::meshingBoundaryConditionInletUniform(const geometry
                     ++num;
                 }
             }
-    if (verbose) std::cout << "\nMeshed cuboid id " << cuboid.id << " as Inlet.\nBoundary vertexes number: " << num << "\n";
-    return true;
+    return num;
 }
 
 // meshingBoundaryConditionOutlet (:276-325)

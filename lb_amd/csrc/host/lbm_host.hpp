@@ -86,6 +86,8 @@ struct LBMConstants {
     RealType CLES = 0.173f;  // the reference's const member (:70); settable here
     RealType omegaDumpingLow = -1.f, omegaDumpingHigh = -1.f;  // LBMConstants.h:74-76
     int outlet_sponge = 0;   // 1: the ...Omega outlets' omega damping (outletOmega)
+    int wall_mesh = 0;       // 1: BounceBackWallHalfMesh (flag-2 cells) instead of BounceBackWallHalf
+    int async_output = 1;    // 1: VTK output from snapshots copied while the next steps run
     // simulation
     RealType err = 0;
     RealType time = 0, plot_every = -1.f, err_every = -1.f;
@@ -194,6 +196,20 @@ public:
     bool meshingBoundaryConditionInletUniform(const geometryObjectCuboid& cuboid, const Vector& normal_out,
                                               const Vector& velocity_in, bool verbose,
                                               int kind = LBM_BC_INLET_EQUILIBRIUM);
+    // meshingBoundaryConditionInletParaboloidCircle / ...Rectangle (:111-274):
+    // the cuboid's cells become inlet nodes (the uniform rule) and every inlet
+    // entry whose centre lies in the cuboid gets the parabolic velocity
+    // against its normal.  The rectangle version sets the extent along the
+    // normal to 1, in the caller's variables too, as the reference does.
+    void meshingBoundaryConditionInletParaboloidCircle(const geometryObjectCuboid& cuboid, const d3& inletCenter,
+                                                       RealType inletRadius, const Vector& normal_out,
+                                                       RealType mean_velocity_in, bool verbose,
+                                                       int kind = LBM_BC_INLET_EQUILIBRIUM);
+    void meshingBoundaryConditionInletParaboloidRectangle(const geometryObjectCuboid& cuboid, const d3& inletCenter,
+                                                          RealType& inletDimX, RealType& inletDimY,
+                                                          RealType& inletDimZ, const Vector& normal_out,
+                                                          RealType mean_velocity_in, bool verbose,
+                                                          int kind = LBM_BC_INLET_EQUILIBRIUM);
     void meshingBoundaryConditionOutlet(const geometryObjectCuboid& cuboid, const Vector& normal_out, RealType density,
                                         bool verbose);
     void meshingBoundaryConditionPeriodic(const geometryObjectCuboid& cuboid, const Vector& normal_out,
@@ -213,6 +229,9 @@ public:
 
 private:
     d3 cellCentre(int i, int j, int k) const;
+    // the uniform rule's cell selection without its velocity check
+    int meshInletCells(const geometryObjectCuboid& cuboid, const Vector& normal_out, const Vector& velocity_in,
+                       int kind);
     LBMConstants& Constants;
     LBMData& Data;
 };
@@ -236,11 +255,16 @@ struct outputerMesh {
 struct outputerVTK {
     static bool MeshVTK(const LBMData& Data, const LBMConstants& Constants, const std::string& filename);
     // physical units (rho*Crho, p = rho cs^2 * Cpressure, u*Cu)
-    static bool variablesVTK(LBMData& Data, const LBMConstants& Constants, int step, bool verbose);
+    // fetch = false: the fields are already in Data.rho_out .. uz_out (a snapshot)
+    static bool variablesVTK(LBMData& Data, const LBMConstants& Constants, int step, bool verbose, bool fetch = true);
     static bool variablesLatticeVTK(LBMData& Data, const LBMConstants& Constants, int step, bool verbose);
     static bool omegaVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
     // variablesTimeAvgVTK (outputerVTK.h:281-345): the time-averaged rho, p, u
     static bool variablesTimeAvgVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
+    // variablesTimeAvgLatticeVTK (:449-576), distributionFunctionVTK (:660-740), tauVTK (:742-822)
+    static bool variablesTimeAvgLatticeVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
+    static bool distributionFunctionVTK(LBMData& Data, const LBMConstants& Constants, int velocity, bool verbose);
+    static bool tauVTK(LBMData& Data, const LBMConstants& Constants, bool verbose);
     static bool probeWriteOut(const LBMConstants& Constants, RealType velX, RealType velY, RealType velZ,
                               RealType rho);
 };

@@ -128,9 +128,9 @@ static bool fetch_fields(LBMData& Data)
 }
 
 static bool write_vars(LBMData& Data, const LBMConstants& C, const std::string& path, double crho, double cp,
-                       double cu, const char* pname)
+                       double cu, const char* pname, bool fetch = true)
 {
-    if (!fetch_fields(Data)) return false;
+    if (fetch && !fetch_fields(Data)) return false;
     std::ofstream out(path);
     if (!out.is_open()) return false;
     vtk_header(out, C, "variables");
@@ -144,11 +144,11 @@ static bool write_vars(LBMData& Data, const LBMConstants& C, const std::string& 
     return (bool)out;
 }
 
-bool outputerVTK::variablesVTK(LBMData& Data, const LBMConstants& C, int step, bool verbose)
+bool outputerVTK::variablesVTK(LBMData& Data, const LBMConstants& C, int step, bool verbose, bool fetch)
 {
     if (verbose) std::cout << "\nDevice -> Host output variables\n";
     return write_vars(Data, C, C.results_dir + "variables." + std::to_string(step) + ".vtk", C.Crho, C.Cpressure, C.Cu,
-                      "p");
+                      "p", fetch);
 }
 
 bool outputerVTK::variablesLatticeVTK(LBMData& Data, const LBMConstants& C, int step, bool verbose)
@@ -176,6 +176,55 @@ bool outputerVTK::variablesTimeAvgVTK(LBMData& Data, const LBMConstants& C, bool
     return (bool)out;
 }
 
+// variablesTimeAvgLatticeVTK (outputerVTK.h:449-576): the time averages in
+// lattice units, pressure rho cs^2.
+bool outputerVTK::variablesTimeAvgLatticeVTK(LBMData& Data, const LBMConstants& C, bool verbose)
+{
+    if (verbose) std::cout << "\nWriting time averaged moments\n\nDevice -> Host output Lattice units\n";
+    const size_t N = Data.meshFluidHost.size();
+    if (Data.rho_avg.size() != N) return false;
+    std::ofstream out(C.results_dir + "variablesTimeAvg.vtk");
+    if (!out.is_open()) return false;
+    vtk_header(out, C, "variables");
+    out << "SCALARS rho double 1\nLOOKUP_TABLE default\n";
+    for (double r : Data.rho_avg) out << r << "\n";
+    out << "SCALARS pressure double 1\nLOOKUP_TABLE default\n";
+    for (double r : Data.rho_avg) out << r * C.cs2 << "\n";
+    out << "VECTORS U double\n";
+    for (size_t i = 0; i < N; ++i) out << Data.ux_avg[i] << " " << Data.uy_avg[i] << " " << Data.uz_avg[i] << "\n";
+    return (bool)out;
+}
+
+// distributionFunctionVTK (outputerVTK.h:660-740): one population plane of
+// the current state (the scalar's name keeps the reference's spelling).
+bool outputerVTK::distributionFunctionVTK(LBMData& Data, const LBMConstants& C, int velocity, bool verbose)
+{
+    if (verbose) std::cout << "\nDevice -> Host output distribution function\n";
+    std::vector<double> f(Data.meshFluidHost.size());
+    if (!Data.ctx || lbm_get_population(Data.ctx, velocity, f.data()) != LBM_OK) return false;
+    std::ofstream out(C.results_dir + "distributionFunction" + std::to_string(velocity) + ".vtk");
+    if (!out.is_open()) return false;
+    vtk_header(out, C, "variables");
+    out << "SCALARS DFunciton_" << velocity << " double 1\nLOOKUP_TABLE default\n";
+    for (double v : f) out << v << "\n";
+    return (bool)out;
+}
+
+// tauVTK (outputerVTK.h:742-822): 1 / omega of the next step's rate, in
+// RealType (inf on solid cells, whose omega is 0).
+bool outputerVTK::tauVTK(LBMData& Data, const LBMConstants& C, bool verbose)
+{
+    if (verbose) std::cout << "\nDevice -> Host output Tau.\n";
+    Data.omega_out.resize(Data.meshFluidHost.size());
+    if (!Data.ctx || lbm_get_omega(Data.ctx, Data.omega_out.data()) != LBM_OK) return false;
+    std::ofstream out(C.results_dir + "tau.vtk");
+    if (!out.is_open()) return false;
+    vtk_header(out, C, "variables");
+    out << "SCALARS Tau double 1\nLOOKUP_TABLE default\n";
+    for (double w : Data.omega_out) out << 1.f / (RealType)w << "\n";
+    return (bool)out;
+}
+
 bool outputerVTK::omegaVTK(LBMData& Data, const LBMConstants& C, bool verbose)
 {
     if (verbose) std::cout << "\nDevice -> Host output omega\n";
@@ -183,7 +232,7 @@ bool outputerVTK::omegaVTK(LBMData& Data, const LBMConstants& C, bool verbose)
     if (!Data.ctx || lbm_get_omega(Data.ctx, Data.omega_out.data()) != LBM_OK) return false;
     std::ofstream out(C.results_dir + "omega.vtk");
     if (!out.is_open()) return false;
-    vtk_header(out, C, "omega");
+    vtk_header(out, C, "variables");
     out << "SCALARS Omega double 1\nLOOKUP_TABLE default\n";
     for (double w : Data.omega_out) out << w << "\n";
     return (bool)out;

@@ -123,6 +123,7 @@ int SolverTurbulentLES::initializeSimulation(bool verbose)
     d.nu = Constants.ny;
     d.c_les = Constants.CLES;
     d.outlet_sponge = Constants.outlet_sponge;
+    d.wall_mesh = Constants.wall_mesh;
     d.sponge_low = (float)Constants.omegaDumpingLow;
     d.sponge_high = (float)Constants.omegaDumpingHigh;
     d.rho0 = Constants.rho_fyz / Constants.Cm * Constants.Cl * Constants.Cl * Constants.Cl;
@@ -255,17 +256,43 @@ int SolverTurbulentLES::runSimulation()
     // straddles its start, so each batch is either counted or not
     const int avg_from = iters - Constants.iterationsMomentAvg;
     int status = LBM_OK;
+    // VTK output at the plot cadence: with async_output the fields are
+    // snapshotted behind the queued steps (lbm_snapshot_begin) and the file
+    // is written after the next batch of steps has been queued, while the
+    // device runs it; otherwise the loop waits for the device at each plot
+    const bool async_out = Constants.async_output != 0;
+    int pending_plot = -1;
+    auto write_pending = [&]() -> int {
+        if (pending_plot < 0) return LBM_OK;
+        const size_t N = Data.meshFluidHost.size();
+        Data.rho_out.resize(N);
+        Data.ux_out.resize(N);
+        Data.uy_out.resize(N);
+        Data.uz_out.resize(N);
+        timer_output.start();
+        int st = lbm_snapshot_wait(Data.ctx, Data.rho_out.data(), Data.ux_out.data(), Data.uy_out.data(),
+                                   Data.uz_out.data(), nullptr, nullptr);
+        if (st == LBM_OK) outputerVTK::variablesVTK(Data, Constants, pending_plot, true, false);
+        timer_output.stop();
+        pending_plot = -1;
+        return st;
+    };
     while (k < iters) {
         // advance to the end of step k, or further while no host event is due
         int kk = k;
         while (kk + 1 < iters && !is_event(kk) && kk != avg_from) ++kk;
         timer_collision.start();
         status = k > avg_from ? lbm_step_moment_add(Data.ctx, kk - k + 1) : lbm_step(Data.ctx, kk - k + 1);
+        timer_collision.stop();
+        // the previous plot's file, while the device runs the steps just queued
+        if (status == LBM_OK) status = write_pending();
         // wait for the device only where the host reads fields (error norm,
-        // output) and at the end; probe records need no wait
+        // synchronous output) and at the end; probe records need no wait
         const bool err_due = Constants.err_every_it > 0 && kk % Constants.err_every_it == 0 && kk != 0;
         const bool plot_due = write_output && Constants.plot_every_it > 0 && kk % Constants.plot_every_it == 0 && kk;
-        if (status == LBM_OK && (err_due || plot_due || kk + 1 == iters)) status = lbm_sync(Data.ctx);
+        timer_collision.start();
+        if (status == LBM_OK && (err_due || (plot_due && !async_out) || kk + 1 == iters))
+            status = lbm_sync(Data.ctx);
         timer_collision.stop();
         if (status != LBM_OK) break;
         k = kk;
@@ -294,11 +321,21 @@ int SolverTurbulentLES::runSimulation()
             }
         }
         if (write_output && Constants.plot_every_it > 0 && k % Constants.plot_every_it == 0 && k) {
-            timer_output.start();
-            outputerVTK::variablesVTK(Data, Constants, k / Constants.plot_every_it, true);
-            timer_output.stop();
+            if (async_out) {
+                status = lbm_snapshot_begin(Data.ctx);
+                if (status != LBM_OK) break;
+                pending_plot = k / Constants.plot_every_it;
+            } else {
+                timer_output.start();
+                outputerVTK::variablesVTK(Data, Constants, k / Constants.plot_every_it, true);
+                timer_output.stop();
+            }
         }
         k++;
+    }
+    {
+        const int st = write_pending();   // a plot at the last step, or before a break
+        if (status == LBM_OK) status = st;
     }
     if (status == LBM_OK || status == LBM_E_NAN) {
         const int st = probeFlush();

@@ -88,6 +88,14 @@ struct lbm_ctx {
     void* uerr = nullptr;           // previous velocity, 3 planes of the arithmetic type, caller layout
     // per-block partials of a diagnostics step go here instead of `partials` when set
     double* diag_dst = nullptr;
+    // asynchronous snapshots: rho, u, omega into snap_dev on the context stream,
+    // then a copy to pinned host memory on copy_stream while steps go on
+    double* snap_dev = nullptr;       // 5 planes FP64, caller layout
+    double* snap_host = nullptr;      // pinned, 5 planes
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_snap = nullptr, ev_snap_copy = nullptr;
+    bool snap_pending = false;
+    int64_t snap_step = 0;
 };
 
 static constexpr int PROBE_CAP = 4096;
@@ -481,6 +489,10 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
         return LBM_OK;
     };
 
+    if (d->wall_mesh != 0 && d->wall_mesh != 1) return LBM_E_ARG;
+    // BounceBackWallHalfMesh (BounceBackWallHalfMesh.h:38-50): every cell with flag 2
+    if (d->wall_mesh)
+        for (size_t i = 0; i < NU; ++i) wall[i] = d->flags[i] == 2 ? 1 : 0;
     for (int n = 0; n < d->n_bc; ++n) {
         const lbm_bc_node& b = d->bc[n];
         if (b.x < 0 || b.y < 0 || b.z < 0 || b.x >= L.ux || b.y >= L.uy || b.z >= L.uz) return LBM_E_ARG;
@@ -489,7 +501,7 @@ static lbm_status classify(const lbm_desc* d, const Layout& L, Plan& P)
         switch (b.kind) {
         case LBM_BC_WALL:
             if (F <= 0) return LBM_E_ARG;  // wall list holds fluid nodes only (geometryMesherBoundary.h:540-542)
-            wall[i] = 1;
+            if (!d->wall_mesh) wall[i] = 1;
             break;
         case LBM_BC_INLET_MOVING_WALL:
         case LBM_BC_INLET_ZOU_HE:
@@ -764,9 +776,14 @@ void lbm_destroy(lbm_ctx* c)
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
                     c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
                     c->cl_wb,    c->ccache,   c->avg,   c->probe_buf, c->inl_post, c->inl_mac,
-                    c->flag1,    c->uerr};
+                    c->flag1,    c->uerr,     c->snap_dev};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->snap_pending && c->ev_snap_copy) (void)hipEventSynchronize(c->ev_snap_copy);
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    if (c->ev_snap) (void)hipEventDestroy(c->ev_snap);
+    if (c->ev_snap_copy) (void)hipEventDestroy(c->ev_snap_copy);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1108,12 +1125,14 @@ lbm_status lbm_sync(lbm_ctx* c)
     return c->nan_seen ? LBM_E_NAN : LBM_OK;
 }
 
-// rho, u, omega of the current state into the 5 staging planes.
-static lbm_status run_macro(lbm_ctx* c)
+// rho, u, omega of the current state into 5 FP64 planes (the staging
+// planes unless dst is given).
+static lbm_status run_macro(lbm_ctx* c, double* dst = nullptr)
 {
     return with_storage(c, [&](auto stv) -> lbm_status {
         using ST = decltype(stv);
         AuxArgs<ST> a = aux_args<ST>(c);
+        if (dst) a.io = dst;
         k_macro<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
         HIPCHK(hipGetLastError());
         return LBM_OK;
@@ -1250,6 +1269,79 @@ lbm_status lbm_get_omega(lbm_ctx* c, double* omega)
     }
     if (st != LBM_OK) return st;
     HIPCHK(hipMemcpyAsync(omega, c->stage + 4 * c->NU, c->NU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+lbm_status lbm_snapshot_begin(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    if (c->snap_pending) return LBM_E_STATE;
+    if (c->d.closure == LBM_CLOSURE_MLP && !c->mlp_ready) return LBM_E_STATE;
+    if (!c->snap_dev) {
+        lbm_status st = dev_alloc(c, (void**)&c->snap_dev, 5 * c->NU * sizeof(double));
+        if (st != LBM_OK) return st;
+        if (hipHostMalloc((void**)&c->snap_host, 5 * c->NU * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            c->snap_host = nullptr;
+            return LBM_E_OOM;
+        }
+        if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_snap_copy, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return LBM_E_HIP;
+        }
+    }
+    // the previous snapshot's copy has finished (it was collected), so
+    // snap_dev is free; the planes are formed behind the queued steps
+    lbm_status st = run_macro(c, c->snap_dev);
+    if (st != LBM_OK) return st;
+    if (c->d.closure == LBM_CLOSURE_MLP) {
+        st = run_closure_eval(c, nullptr, nullptr, nullptr, c->snap_dev + 4 * c->NU);
+        if (st != LBM_OK) return st;
+    }
+    HIPCHK(hipEventRecord(c->ev_snap, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->copy_stream, c->ev_snap, 0));
+    HIPCHK(hipMemcpyAsync(c->snap_host, c->snap_dev, 5 * c->NU * sizeof(double), hipMemcpyDeviceToHost,
+                          c->copy_stream));
+    HIPCHK(hipEventRecord(c->ev_snap_copy, c->copy_stream));
+    c->snap_pending = true;
+    c->snap_step = c->steps;
+    return LBM_OK;
+}
+
+lbm_status lbm_snapshot_wait(lbm_ctx* c, double* rho, double* ux, double* uy, double* uz, double* omega,
+                             int64_t* step)
+{
+    if (!c) return LBM_E_ARG;
+    if (!c->snap_pending) return LBM_E_STATE;
+    HIPCHK(hipEventSynchronize(c->ev_snap_copy));
+    c->snap_pending = false;
+    double* outs[5] = {rho, ux, uy, uz, omega};
+    for (int k = 0; k < 5; ++k)
+        if (outs[k]) std::memcpy(outs[k], c->snap_host + (size_t)k * c->NU, c->NU * sizeof(double));
+    if (step) *step = c->snap_step;
+    return LBM_OK;
+}
+
+lbm_status lbm_get_population(lbm_ctx* c, int32_t q, double* f)
+{
+    if (!c || !f || q < 0 || q >= Q) return LBM_E_ARG;
+    if (!c->initialized) return LBM_E_STATE;
+    lbm_status st = lbm_sync(c);
+    if (st != LBM_OK && st != LBM_E_NAN) return st;
+    st = with_storage(c, [&](auto stv) -> lbm_status {
+        using ST = decltype(stv);
+        AuxArgs<ST> a = aux_args<ST>(c);
+        a.q = q;
+        k_gather<ST><<<blocks_for(c->NU), TPB, 0, c->stream>>>(a);
+        HIPCHK(hipGetLastError());
+        return LBM_OK;
+    });
+    if (st != LBM_OK) return st;
+    HIPCHK(hipMemcpyAsync(f, c->stage, c->NU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return LBM_OK;
 }

@@ -31,6 +31,10 @@ def load_host() -> ctypes.CDLL:
                                                   ctypes.POINTER(vp)]),
             "lbmh_mesher_wall": (ctypes.c_int, [vp]),
             "lbmh_mesher_inlet": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, f3, ctypes.c_int32]),
+            "lbmh_mesher_inlet_paraboloid_circle": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, d3,
+                                                                   ctypes.c_float, ctypes.c_float, ctypes.c_int32]),
+            "lbmh_mesher_inlet_paraboloid_rectangle": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, d3, f3,
+                                                                      ctypes.c_float, ctypes.c_int32]),
             "lbmh_mesher_outlet": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, ctypes.c_float]),
             "lbmh_mesher_periodic": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, ctypes.c_int32]),
             "lbmh_mesher_compile": (ctypes.c_int, [vp, i32p, ctypes.POINTER(abi.BcNode), i32p]),
@@ -43,6 +47,10 @@ def load_host() -> ctypes.CDLL:
             "lbmh_case_set_grid": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_float, ctypes.c_float, f3, f3]),
             "lbmh_case_mesh_wall": (ctypes.c_int, [vp]),
             "lbmh_case_inlet": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, f3, ctypes.c_int32]),
+            "lbmh_case_inlet_paraboloid_circle": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, d3,
+                                                                 ctypes.c_float, ctypes.c_float, ctypes.c_int32]),
+            "lbmh_case_inlet_paraboloid_rectangle": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, d3, f3,
+                                                                    ctypes.c_float, ctypes.c_int32]),
             "lbmh_case_outlet": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, ctypes.c_float]),
             "lbmh_case_periodic": (ctypes.c_int, [vp, d3, d3, d3, ctypes.c_int32, f3, ctypes.c_int32]),
             "lbmh_case_compile": (ctypes.c_int, [vp]),
@@ -55,6 +63,7 @@ def load_host() -> ctypes.CDLL:
             "lbmh_case_time_avg": (ctypes.c_int, [vp, d3, d3, d3, d3, ctypes.POINTER(ctypes.c_int64)]),
             "lbmh_read_vtk_fields": (ctypes.c_int, [ctypes.c_char_p, i32p, d3, d3, d3, d3]),
             "lbmh_case_set_init_file": (ctypes.c_int, [vp, ctypes.c_char_p]),
+            "lbmh_case_write_vtk": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int32]),
             "lbmh_case_destroy": (None, [vp]),
         }
         for k, (r, a) in sig.items():
@@ -139,6 +148,21 @@ class Mesher:
 
     def inlet(self, p1, p2, p3, cid, normal, velocity, kind=abi.BC_INLET_EQUILIBRIUM):
         return self._L.lbmh_mesher_inlet(self._h, _d3(p1), _d3(p2), _d3(p3), cid, _f3(normal), _f3(velocity), kind)
+
+    def inlet_paraboloid_circle(self, p1, p2, p3, cid, normal, center, radius, mean_velocity,
+                                kind=abi.BC_INLET_EQUILIBRIUM):
+        return self._L.lbmh_mesher_inlet_paraboloid_circle(self._h, _d3(p1), _d3(p2), _d3(p3), cid, _f3(normal),
+                                                           _d3(center), radius, mean_velocity, kind)
+
+    def inlet_paraboloid_rectangle(self, p1, p2, p3, cid, normal, center, dims, mean_velocity,
+                                   kind=abi.BC_INLET_EQUILIBRIUM):
+        """Returns the inlet extents after the call (the one along the normal becomes 1)."""
+        d = (ctypes.c_float * 3)(*dims)
+        st = self._L.lbmh_mesher_inlet_paraboloid_rectangle(self._h, _d3(p1), _d3(p2), _d3(p3), cid, _f3(normal),
+                                                            _d3(center), d, mean_velocity, kind)
+        if st:
+            raise RuntimeError("lbmh_mesher_inlet_paraboloid_rectangle failed")
+        return tuple(d)
 
     def outlet(self, p1, p2, p3, cid, normal, density):
         return self._L.lbmh_mesher_outlet(self._h, _d3(p1), _d3(p2), _d3(p3), cid, _f3(normal), density)

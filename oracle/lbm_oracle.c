@@ -135,6 +135,7 @@ int orc_get_threads(void)
         (S)->collision = d->collision; (S)->closure = d->closure; (S)->compat = compat;     \
         (S)->tau = (T)d->tau; (S)->nu = (T)d->nu; (S)->CLES = (T)d->c_les;                  \
         (S)->sponge = d->outlet_sponge != 0;                                                \
+        (S)->wall_mesh = d->wall_mesh != 0;                                                 \
         (S)->sponge_low = (T)d->sponge_low; (S)->sponge_high = (T)d->sponge_high;           \
         ORC_FILL_CONSTS(S, T);                                                              \
         (S)->flags = (int*)malloc(sizeof(int) * N);                                         \

@@ -25,6 +25,7 @@ typedef struct FN(state) {
     REAL tau, nu, CLES, c13, cs;
     FN(ck) ck;
     int sponge;             /* outlet omega damping on */
+    int wall_mesh;          /* BounceBackWallHalfMesh instead of the wall list */
     REAL sponge_low, sponge_high;
     REAL w[27];
     int crev[27];
@@ -480,6 +481,22 @@ static void FN(op_streaming)(FN(state) * s)
 /* BounceBackWallHalf::bounceBackWall (BounceBackWallHalf.h:29-52) */
 static void FN(op_bounce_back)(FN(state) * s)
 {
+    if (s->wall_mesh) {
+        /* BounceBackWallHalfMesh::bounceBackWall (BounceBackWallHalfMesh.h:38-50):
+         * every cell whose flag is 2 */
+        for (int z = 0; z < s->nz; ++z)
+            for (int y = 0; y < s->ny; ++y)
+                for (int x = 0; x < s->nx; ++x) {
+                    long i = FN(idx)(s, x, y, z);
+                    if (s->flags[i] != 2) continue;
+                    for (int vel = 0; vel < 27; ++vel) {
+                        long j;
+                        if (!FN(nb)(s, x - ORC_C[vel][0], y - ORC_C[vel][1], z - ORC_C[vel][2], &j)) continue;
+                        if (s->flags[j] == 0) DF(s, vel, i) = DP(s, s->crev[vel], i);
+                    }
+                }
+        return;
+    }
     for (int n = 0; n < s->n_bc; ++n) {
         const lbm_bc_node* b = &s->bc[n];
         if (b->kind != LBM_BC_WALL) continue;

@@ -9,6 +9,8 @@ meshFluidHost with index_sequence<2,1,0>.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 # lbm_bc_node kinds (include/lbm_core.h)
@@ -83,6 +85,73 @@ def meshing_inlet_uniform(flags, mask, cuboid_id, normal, velocity):
         flags[z, y, x] = cuboid_id
         nodes.append(dict(x=x, y=y, z=z, normal=tuple(normal), velocity=tuple(velocity), regular=reg))
     return nodes
+
+
+def _inside(p1, p2, p3, q):
+    lo = [min(p1[a], p2[a], p3[a]) for a in range(3)]
+    hi = [max(p1[a], p2[a], p3[a]) for a in range(3)]
+    return all(lo[a] <= q[a] <= hi[a] for a in range(3))
+
+
+def _centre32(d, resolution, additional, bbmin):
+    """Cell centre of an entry held in float (RealType lookx), as floats."""
+    c = cell_centre(d["x"], d["y"], d["z"], resolution, additional, bbmin)
+    return tuple(float(np.float32(v)) for v in c)
+
+
+def _set_velocity(d, v, normal):
+    f = np.float32
+    d["velocity"] = tuple(float(f(-v) * f(n)) for n in normal)
+
+
+def meshing_inlet_paraboloid_circle(flags, inlets, p1, p2, p3, cuboid_id, normal, center, radius, mean,
+                                    resolution=1.0, additional=1.0, bbmin=(0.0, 0.0, 0.0)):
+    """meshingBoundaryConditionInletParaboloidCircle (:111-178): the uniform
+    rule with velocity 0 appends the cuboid's cells to `inlets`, then every
+    entry whose float centre is in the cuboid and within `radius` of `center`
+    gets v = 3 mean (1 - (r/R)^2) against the normal (r in double, held in
+    float; the square in double)."""
+    f = np.float32
+    mask = cuboid_mask(flags.shape, p1, p2, p3, resolution, additional, bbmin, lookup_dtype=np.float32)
+    inlets += meshing_inlet_uniform(flags, mask, cuboid_id, normal, (0.0, 0.0, 0.0))
+    for d in inlets:
+        q = _centre32(d, resolution, additional, bbmin)
+        if not _inside(p1, p2, p3, q):
+            continue
+        dist = f(math.sqrt(math.pow(q[0] - center[0], 2) + math.pow(q[1] - center[1], 2) +
+                           math.pow(q[2] - center[2], 2)))
+        if dist <= f(radius):
+            v = f(float(f(3) * f(mean)) * (1.0 - math.pow(float(dist / f(radius)), 2)))
+            _set_velocity(d, max(v, f(0)), normal)
+    return inlets
+
+
+def meshing_inlet_paraboloid_rectangle(flags, inlets, p1, p2, p3, cuboid_id, normal, center, dims, mean,
+                                       resolution=1.0, additional=1.0, bbmin=(0.0, 0.0, 0.0)):
+    """meshingBoundaryConditionInletParaboloidRectangle (:180-274): a product of
+    1-D parabolas d_i (L_i - d_i) over the axes across the normal, divided by
+    prod (L_i / 2)^2 (float, the reference's operation order), times 2 mean.
+    Returns the inlets and the extents, the one along the normal set to 1."""
+    f = np.float32
+    L = [f(v) for v in dims]
+    mask = cuboid_mask(flags.shape, p1, p2, p3, resolution, additional, bbmin, lookup_dtype=np.float32)
+    inlets += meshing_inlet_uniform(flags, mask, cuboid_id, normal, (0.0, 0.0, 0.0))
+    for d in inlets:
+        q = _centre32(d, resolution, additional, bbmin)
+        if not _inside(p1, p2, p3, q):
+            continue
+        part = []
+        for a in range(3):
+            dist = f(q[a] - (center[a] - float(L[a] / f(2))))
+            if normal[a] == 0:
+                part.append(dist * (L[a] - dist))
+            else:
+                part.append(f(1))
+                L[a] = f(1)
+        const = L[0] * L[1] * L[2] * L[0] * L[1] * L[2] / f(16)
+        v = f(2) * f(mean) * part[0] * part[1] * part[2] / const
+        _set_velocity(d, max(v, f(0)), normal)
+    return inlets, tuple(float(v) for v in L)
 
 
 def meshing_outlet(flags, mask, cuboid_id, normal, density):

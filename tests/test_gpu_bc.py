@@ -143,3 +143,25 @@ def test_uniform_flow_channel_steady_on_gpu(gpu):
     rho, ux, uy, uz = g.macroscopic()
     assert np.abs(rho - 1).max() < 1e-14 and np.abs(ux - u0).max() < 1e-14
     assert np.abs(uy).max() < 1e-14 and np.abs(uz).max() < 1e-14
+
+
+@pytest.mark.parametrize("storage,arith", [(abi.STORAGE_F64, abi.ARITH_LITERAL), (abi.STORAGE_F32, abi.ARITH_LITERAL)],
+                         ids=["f64", "f32"])
+def test_wall_mesh_policy_equals_oracle(gpu, storage, arith):
+    """BounceBackWallHalfMesh (BounceBackWallHalfMesh.h:38-50): half-way
+    bounce-back on the flag-2 cells around isolated solid cells and a block
+    in a periodic box, the wall list ignored; literal arithmetic bit for bit."""
+    from test_oracle import _sparse_obstacles
+    flags, walls = _sparse_obstacles(16)
+    bc = bc_nodes([dict(kind=mo.BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls])
+    u = random_velocity_field(flags.shape, 0.03, seed=14)
+    init = (None, *[np.where(flags == 0, 0.0, a) for a in u])
+    g, o = _run_pair(flags, bc, 40, init, oracle_storage=storage, collision=abi.COLLISION_CUM2015,
+                     closure=abi.CLOSURE_SMAGORINSKY, c_les=0.17, storage=storage, arith=arith,
+                     compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=0.56, wall_mesh=True)
+    fg, fo = g.populations(), o.populations()
+    print(f"[wall mesh {abi.STORAGE_NAMES[storage]}] max|df| {np.abs(fg - fo).max():.3e}", flush=True)
+    assert np.array_equal(fg, fo)
+    # and the drag diagnostic sees the flag-2 links as the walls
+    _, _, force = g.step_diagnostics()
+    assert np.all(np.isfinite(force)) and np.abs(force).max() > 0

@@ -2,6 +2,7 @@
 mesher, unit conversion.  Checked against the reference's committed
 voxelization, the oracle mesher / brute-force voxelizer, and analytic shapes."""
 import gzip
+import math
 import os
 
 import numpy as np
@@ -253,3 +254,85 @@ def test_vtk_initial_fields_reader(tmp_path):
     short_r.write_text("\n".join(text[:15]) + "\n")
     assert L.lbmh_read_vtk_fields(str(short_r).encode(), d, *[abi.ptr(a) for a in out]) == 2
     assert L.lbmh_read_vtk_fields(str(tmp_path / "none.vtk").encode(), d, *[abi.ptr(a) for a in out]) == 1
+
+
+# --- parabolic inlets (geometryMesherBoundary.h:111-274) -------------------
+
+def _pipe_grid(nx=6, ny=17, nz=17):
+    """A duct along x: solid rows on the y and z faces, fluid inside."""
+    g = np.zeros((nz, ny, nx), np.int32)
+    g[1:-1, 1:-1, :] = 1
+    return g
+
+
+def _inlet_lists(m, of, obc):
+    flags, bc = m.compile()
+    assert np.array_equal(flags, of)
+    got = [(b.x, b.y, b.z, b.ux, b.uy, b.uz) for b in bc if b.kind == abi.BC_INLET_EQUILIBRIUM]
+    want = [(d["x"], d["y"], d["z"], *[float(np.float32(v)) for v in d["velocity"]]) for d in obc]
+    return got, want, bc
+
+
+def test_paraboloid_circle_inlet_matches_restatement_and_closed_form():
+    g = _pipe_grid()
+    nz, ny, nx = g.shape
+    # additional 1: cell j has centre j - 0.5; the duct's interior spans y, z in [0, 15];
+    # the centre lies in the plane of the inlet cells' centres (the distance is 3-D)
+    centre, R, U = (-0.5, 7.5, 7.5), 7.0, 0.04
+    p1, p2, p3 = (-0.9, -1.0, -1.0), (-0.1, 20.0, 20.0), (-0.1, 20.0, -1.0)
+    m = host.Mesher(g, 1.0, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    of, _ = mo.meshing_boundary_wall(g)
+    assert m.inlet_paraboloid_circle(p1, p2, p3, -1, (-1.0, 0.0, 0.0), centre, R, U) == 0
+    obc = mo.meshing_inlet_paraboloid_circle(of, [], p1, p2, p3, -1, (-1.0, 0.0, 0.0), centre, R, U)
+    got, want, bc = _inlet_lists(m, of, obc)
+    assert got == want and len(got) == (ny - 2) * (nz - 2)
+    # closed form: u_x = 3 U (1 - r^2 / R^2) inside R, 0 outside, into the domain (+x)
+    for (x, y, z, ux, uy, uz) in got:
+        r = math.hypot(y - 0.5 - centre[1], z - 0.5 - centre[2])
+        expect = 3 * U * (1 - r * r / (R * R)) if r <= R else 0.0
+        assert abs(ux - expect) < 1e-7 and uy == 0.0 and uz == 0.0
+    # the cell at the centre carries the peak 3 U: the profile's mean over the
+    # disc is 1.5 U (a pipe's Poiseuille peak would be 2 U)
+    assert max(t[3] for t in got) == pytest.approx(3 * U, rel=1e-6)
+
+
+def test_paraboloid_rectangle_inlet_matches_restatement_and_closed_form():
+    g = _pipe_grid(nx=6, ny=13, nz=21)
+    centre, dims, U = (0.0, 5.5, 9.5), (3.0, 11.0, 19.0), 0.03
+    p1, p2, p3 = (-0.9, -1.0, -1.0), (-0.1, 30.0, 30.0), (-0.1, 30.0, -1.0)
+    m = host.Mesher(g, 1.0, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    of, _ = mo.meshing_boundary_wall(g)
+    dims_after = m.inlet_paraboloid_rectangle(p1, p2, p3, -1, (-1.0, 0.0, 0.0), centre, dims, U)
+    obc, dims_o = mo.meshing_inlet_paraboloid_rectangle(of, [], p1, p2, p3, -1, (-1.0, 0.0, 0.0), centre, dims, U)
+    got, want, _ = _inlet_lists(m, of, obc)
+    assert got == want and dims_after == dims_o == (1.0, 11.0, 19.0)
+    # closed form: 2 U * prod 4 d (L - d) / L^2 over the two cross axes, d from the low corner
+    for (x, y, z, ux, uy, uz) in got:
+        dy, dz = (y - 0.5) - (centre[1] - dims[1] / 2), (z - 0.5) - (centre[2] - dims[2] / 2)
+        expect = 2 * U * (4 * dy * (dims[1] - dy) / dims[1] ** 2) * (4 * dz * (dims[2] - dz) / dims[2] ** 2)
+        assert abs(ux - max(expect, 0.0)) < 1e-7
+    # the cell at the centre carries the peak 2 U
+    assert max(t[3] for t in got) == pytest.approx(2 * U, rel=1e-6)
+
+
+def test_paraboloid_overwrites_earlier_inlets_inside_its_cuboid():
+    """The reference loops over every inlet entry, not only the new ones: an
+    earlier uniform inlet inside the cuboid takes the parabolic velocity."""
+    g = _pipe_grid()
+    p1, p2, p3 = (-0.9, -1.0, -1.0), (-0.1, 20.0, 20.0), (-0.1, 20.0, -1.0)
+    m = host.Mesher(g, 1.0, 1.0, (0.0, 0.0, 0.0))
+    m.wall()
+    of, _ = mo.meshing_boundary_wall(g)
+    assert m.inlet(p1, p2, p3, -1, (-1.0, 0.0, 0.0), (0.01, 0.0, 0.0), abi.BC_INLET_EQUILIBRIUM) == 0
+    mask = mo.cuboid_mask(g.shape, p1, p2, p3, lookup_dtype=np.float32)
+    obc = mo.meshing_inlet_uniform(of, mask, -1, (-1.0, 0.0, 0.0), (0.01, 0.0, 0.0))
+    m.inlet_paraboloid_circle(p1, p2, p3, -4, (-1.0, 0.0, 0.0), (-0.5, 7.5, 7.5), 7.0, 0.04)
+    obc = mo.meshing_inlet_paraboloid_circle(of, obc, p1, p2, p3, -4, (-1.0, 0.0, 0.0), (-0.5, 7.5, 7.5), 7.0, 0.04)
+    got, want, _ = _inlet_lists(m, of, obc)
+    assert got == want
+    for (x, y, z, ux, uy, uz) in got:
+        inside = math.hypot(y - 0.5 - 7.5, z - 0.5 - 7.5) <= 7.0
+        assert (ux != np.float32(0.01)) == inside      # outside the radius the old velocity stays
+    assert np.all(of[:, :, 0][g[:, :, 0] == 1] == -1)    # the cells keep the first inlet's id

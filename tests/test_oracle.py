@@ -540,3 +540,42 @@ def test_zou_he_inlet_keeps_a_uniform_inflow():
     o2.step(30)
     inlet = flags == -1
     assert np.all(o2.macroscopic()[1][inlet] > 0.25 * u0)
+
+
+# --- BounceBackWallHalfMesh -----------------------------------------------
+
+def _sparse_obstacles(n=12):
+    """A periodic box with a few solid cells and a small solid block: their
+    26-neighbours carry flags 2 (one solid neighbour) and higher."""
+    from oracle import mesher_oracle as mo
+    g = np.ones((n, n, n), np.int32)
+    g[3, 4, 5] = 0
+    g[8, 2, 9] = 0
+    g[6:8, 7:9, 2:4] = 0
+    flags, walls = mo.meshing_boundary_wall(g)
+    walls = mo.compile_wall(flags, walls)
+    return flags, walls
+
+
+def test_wall_mesh_policy_is_the_list_rule_on_flag_two_cells():
+    """BounceBackWallHalfMesh (BounceBackWallHalfMesh.h:38-50) applies the
+    half-way rule of BounceBackWallHalf to every flag-2 cell: it equals the
+    list policy with the list replaced by exactly those cells."""
+    from helpers import bc_nodes
+    flags, walls = _sparse_obstacles()
+    assert (flags == 2).sum() > 20 and len(walls) > 0
+    listed = bc_nodes([dict(kind=abi.BC_WALL, x=w[0], y=w[1], z=w[2], regular=w[3]) for w in walls])
+    flag2 = [dict(kind=abi.BC_WALL, x=int(x), y=int(y), z=int(z), regular=1)
+             for z, y, x in zip(*np.nonzero(flags == 2))]
+    u = random_velocity_field(flags.shape, 0.03, seed=13)
+    u = [np.where(flags == 0, 0.0, a) for a in u]
+    kw = dict(collision=abi.COLLISION_CUM2015, storage=abi.STORAGE_F64, arith=abi.ARITH_LITERAL,
+              compat=abi.COMPAT_REFERENCE, periodic=(1, 1, 1), tau=0.56)
+    out = []
+    for bc, mesh in ((listed, True), (bc_nodes(flag2), False), (listed, False)):
+        o = OracleLattice(make_desc(flags, bc, wall_mesh=mesh, **kw))
+        o.init_equilibrium(None, *u)
+        o.step(12)
+        out.append(o.populations())
+    assert np.array_equal(out[0], out[1])
+    assert not np.array_equal(out[0], out[2])   # the two policies act on different cells
