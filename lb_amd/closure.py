@@ -375,9 +375,12 @@ if __name__ == "__main__":
     ap.add_argument("--eval-steps", type=int, default=4000)
     ap.add_argument("--adam-steps", type=int, default=3000)
     ap.add_argument("--refresh", type=int, default=10)
+    ap.add_argument("--f-eval", type=int, default=2, help="filter width of the evaluation (LES at fine/f)")
+    ap.add_argument("--eval-every", type=int, default=100)
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
     snaps = tuple(int(a.train_steps * k / 3) for k in (1, 2, 3))
     r = run_experiment(fine_n=a.fine, train_steps=a.train_steps, train_snaps=snaps, eval_steps=a.eval_steps,
-                       adam_steps=a.adam_steps, refresh=a.refresh, out_json=a.out)
+                       eval_every=a.eval_every, adam_steps=a.adam_steps, refresh=a.refresh, out_json=a.out,
+                       f_eval=a.f_eval)
     print(json.dumps(r))
