@@ -52,7 +52,9 @@ enum : uint8_t {
     CL_BND = 2,    // streamed, some neighbour solid, inlet or outside (masks in the node word),
                    // or a cell the Combined operator collides with its 2017 branch
     CL_INLET = 3,  // inlet node (flag -1): own state in a side buffer
-    CL_PUB = 4     // streamed like CL_BND; also publishes rho, u_x and f* for a patch kernel
+    CL_PUB = 4,    // streamed like CL_BND; also publishes rho, u_x and f* for a patch kernel
+    CL_OUTL = 5    // interpolated outlet node whose rule its next reader applies; publishes
+                   // like CL_PUB through its entry's slot (bcidx = outlet entry)
 };
 
 constexpr int TPB = 256;
@@ -101,6 +103,8 @@ struct StepArgs {
     const NodeDev* inl;            // [n_inlet]
     typename ST::R* pub_mac;       // [n_pub][PUB_MAC]: rho, u_x, u_y, u_z
     typename ST::R* pub_post;      // [n_pub][27]: post-collision populations
+    const typename ST::R* pub_prev;  // the previous step's records (deferred outlet)
+    const NodeDev* outl;           // outlet entries (deferred outlet)
     const typename ST::R* macro;   // rho, ux, uy, uz planes for the first step
     double* partials;              // [ntiles][8] when diag != 0: mass, momentum, wall force
     float* ccache;                 // learned closure: coefficient per cell, held between refreshes
@@ -117,6 +121,8 @@ struct StepArgs {
     uint32_t seed;                 // per-step seed of the stochastic-rounding stores
     int sp_axis, sp_begin;         // outlet omega damping: axis (-1 off) and first cell
     typename ST::R sp_deriv;       // omega decrement per cell past sp_begin
+    int outl_pending;              // CL_OUTL cells' loaded state still needs the outlet rule
+    typename ST::R cs;             // the outlet's interpolation weight
 };
 
 // Outlet omega damping (OutletNeighbourEquilibriumOmega.h:18-79) of a cell at
@@ -129,6 +135,76 @@ __device__ __forceinline__ R sponge(R omega, int axis, int begin, R deriv, const
     const int v = axis == 0 ? x - g.px : axis == 1 ? y - g.py : z - g.pz;
     if (v < begin) return omega;
     return omega - deriv * (R)(v - begin);
+}
+
+// OutletDensityInterpolated::outlet (OutletDensityInterpolated.h:170-252) on
+// one node: interpolated incoming populations, then the density reset.
+// The interpolated rule itself on the node's current populations d (the
+// state after streaming), with pub the post-collision records of the step
+// that produced that state.  b.partner holds the irregular +x node's copy
+// mask (host-computed: directions whose cell (x, y - c_y, z - c_z) is solid),
+// applied in q order as the reference's loop does.
+template <class R>
+__device__ __forceinline__ bool outlet_blend(R* d, const NodeDev& b, const R* pub, R cs, bool crev20, R& rho, R& ux,
+                                             R& uy, R& uz)
+{
+    LBM_FP_STRICT
+    const uint32_t mask = (uint32_t)b.partner;
+#pragma unroll
+    for (int vel = 0; vel < Q; ++vel)
+        if ((mask >> vel) & 1u) d[vel] = d[crev_of(vel, crev20)];
+    if (b.s1 == -2) return false;
+    const R* self = pub + (size_t)b.s0 * Q;
+#pragma unroll
+    for (int vel = 0; vel < Q; ++vel) {
+        const float nc = b.nx * (float)CX[vel] + b.ny * (float)CY[vel] + b.nz * (float)CZ[vel];
+        if (nc < 0.0f) {
+            const R up = b.s1 >= 0 ? pub[(size_t)b.s1 * Q + vel] : R(0);
+            d[vel] = cs * up + (R(1) - cs) * self[vel];
+        }
+    }
+    const R density = b.rho;
+    rho = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8] + d[9] + d[10] + d[11] + d[12] + d[13] +
+          d[14] + d[15] + d[16] + d[17] + d[18] + d[19] + d[20] + d[21] + d[22] + d[23] + d[24] + d[25] + d[26];
+    ux = (d[1] - d[2] + d[7] - d[8] + d[9] - d[10] - d[11] + d[12] - d[15] + d[16] - d[19] + d[20] - d[21] + d[22] +
+          d[23] - d[24] - d[25] + d[26]) / density;
+    uy = (-d[5] + d[6] - d[11] + d[12] + d[13] - d[14] + d[15] - d[16] + d[17] - d[18] + d[19] - d[20] - d[21] +
+          d[22] - d[23] + d[24] - d[25] + d[26]) / density;
+    uz = (-d[3] + d[4] - d[7] + d[8] + d[9] - d[10] - d[13] + d[14] + d[17] - d[18] - d[19] + d[20] + d[21] - d[22] -
+          d[23] + d[24] - d[25] + d[26]) / density;
+    return true;
+}
+
+// The density reset of population vel: f - f_eq(u, rho) + f_eq(u, rho_b = 1).
+template <class R>
+__device__ __forceinline__ R outlet_reset(const Consts<R>& K, int vel, R v, R ux, R uy, R uz, R rho)
+{
+    LBM_FP_STRICT
+    return v - feq_q(K, vel, ux, uy, uz, rho) + feq_q(K, vel, ux, uy, uz, R(1));
+}
+
+template <class R>
+__device__ __forceinline__ void interp_outlet(R* d, const NodeDev& b, const R* pub, const Consts<R>& K, R cs,
+                                              bool crev20)
+{
+    R rho, ux, uy, uz;
+    if (outlet_blend(d, b, pub, cs, crev20, rho, ux, uy, uz)) {
+#pragma unroll
+        for (int vel = 0; vel < Q; ++vel) d[vel] = outlet_reset(K, vel, d[vel], ux, uy, uz, rho);
+    }
+}
+
+// The rule as the next reader of a deferred outlet cell applies it: to the
+// populations just loaded, rounded through the storage format as the patch
+// kernel's store and the next load would round them.
+template <class ST>
+__device__ __forceinline__ void deferred_outlet(typename ST::R* f, const NodeDev& b, const typename ST::R* pub,
+                                                const Consts<typename ST::R>& K, typename ST::R cs, bool crev20,
+                                                const typename ST::R* rest)
+{
+    interp_outlet(f, b, pub, K, cs, crev20);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = ST::round_trip(f[q], rest[WCLASS[q]]);
 }
 
 // Arguments of the collide-stream step with the learned closure.
@@ -457,6 +533,26 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
         } else {
             load_pop<ST, ODD>(a.p, a.rest, c, f);
+            // deferred interpolated outlet (4- and 8-byte storage only): the
+            // rule of the previous step applied to the state just loaded
+            if constexpr (sizeof(typename ST::T) >= 4) {
+                if (FLAGS && cls == CL_OUTL && a.outl_pending) {
+                    interp_outlet(f, a.outl[bi], a.pub_prev, a.K, a.cs, a.crev20 != 0);
+                    // incoming populations without a streaming source keep
+                    // their value, which the boundary path reads again from
+                    // the read slot after the collision: the rule's result
+                    // goes there, as the patch kernel would have stored it
+                    // (only this cell reads or writes those slots this step)
+                    const uint64_t nwo = a.node[c.i];
+                    if ((nwo >> 63) == 0) {
+                        const uint32_t So = (uint32_t)(nwo & MASK27);
+#pragma unroll
+                        for (int q = 0; q < Q; ++q)
+                            if ((So >> q) & 1u)
+                                ST::store(a.p[read_plane(ODD, q)], read_index(c, ODD, q), f[q], a.rest[WCLASS[q]]);
+                    }
+                }
+            }
         }
 
         // the Combined operator reads the node word before colliding: bit 62
@@ -598,13 +694,15 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
                     }
                 }
             }
-            // data the boundary patch kernels of this step read
-            if (cls == CL_PUB) {
-                a.pub_mac[(size_t)bi * PUB_MAC] = rho;
-                a.pub_mac[(size_t)bi * PUB_MAC + 1] = ux;
-                a.pub_mac[(size_t)bi * PUB_MAC + 2] = uy;
-                a.pub_mac[(size_t)bi * PUB_MAC + 3] = uz;
-                R* dst = a.pub_post + (size_t)bi * Q;
+            // data the boundary patch kernels of this step (and, for deferred
+            // outlet cells, the next step) read
+            if (cls == CL_PUB || cls == CL_OUTL) {
+                const int ps = cls == CL_PUB ? bi : a.outl[bi].s0;
+                a.pub_mac[(size_t)ps * PUB_MAC] = rho;
+                a.pub_mac[(size_t)ps * PUB_MAC + 1] = ux;
+                a.pub_mac[(size_t)ps * PUB_MAC + 2] = uy;
+                a.pub_mac[(size_t)ps * PUB_MAC + 3] = uz;
+                R* dst = a.pub_post + (size_t)ps * Q;
 #pragma unroll
                 for (int q = 0; q < Q; ++q) dst[q] = P[q];
             }
@@ -930,46 +1028,12 @@ __global__ __launch_bounds__(TPB) void k_outlet(PatchArgs<ST> a)
         outlet_equilibrium(a, b);
         return;
     }
-    const bool xp = b.nx == 1.0f && b.ny == 0.0f && b.nz == 0.0f;
-    const bool xm = b.nx == -1.0f && b.ny == 0.0f && b.nz == 0.0f;
-    if (!xp && !xm) return;
     const Cell c = cell_at(a.g, b.x, b.y, b.z);
     R d[Q];
     int pl[Q];
     uint32_t ix[Q];
     load_node(a, c, d, pl, ix);
-    if (xp && !b.regular) {
-#pragma unroll
-        for (int vel = 0; vel < Q; ++vel) {
-            const int yy = wrap(b.y - CY[vel], a.g.ny, a.g.wy), zz = wrap(b.z - CZ[vel], a.g.nz, a.g.wz);
-            if (on_lattice(a.g, b.x, yy, zz) && a.cls[(uint32_t)(b.x + a.g.nx * (yy + a.g.ny * zz))] == CL_SOLID)
-                d[vel] = d[crev_of(vel, a.crev20 != 0)];
-        }
-    }
-    if (b.s1 != -2) {
-        const R* self = a.pub_post + (size_t)b.s0 * Q;
-#pragma unroll
-        for (int vel = 0; vel < Q; ++vel) {
-            const float nc = b.nx * (float)CX[vel] + b.ny * (float)CY[vel] + b.nz * (float)CZ[vel];
-            if (nc < 0.0f) {
-                const R up = b.s1 >= 0 ? a.pub_post[(size_t)b.s1 * Q + vel] : R(0);
-                d[vel] = a.cs * up + (R(1) - a.cs) * self[vel];
-            }
-        }
-        const R density = b.rho;
-        const R rho = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8] + d[9] + d[10] + d[11] + d[12] +
-                      d[13] + d[14] + d[15] + d[16] + d[17] + d[18] + d[19] + d[20] + d[21] + d[22] + d[23] + d[24] +
-                      d[25] + d[26];
-        const R ux = (d[1] - d[2] + d[7] - d[8] + d[9] - d[10] - d[11] + d[12] - d[15] + d[16] - d[19] + d[20] -
-                      d[21] + d[22] + d[23] - d[24] - d[25] + d[26]) / density;
-        const R uy = (-d[5] + d[6] - d[11] + d[12] + d[13] - d[14] + d[15] - d[16] + d[17] - d[18] + d[19] -
-                      d[20] - d[21] + d[22] - d[23] + d[24] - d[25] + d[26]) / density;
-        const R uz = (-d[3] + d[4] - d[7] + d[8] + d[9] - d[10] - d[13] + d[14] + d[17] - d[18] - d[19] + d[20] +
-                      d[21] - d[22] - d[23] + d[24] - d[25] + d[26]) / density;
-#pragma unroll
-        for (int vel = 0; vel < Q; ++vel)
-            d[vel] = d[vel] - feq_q(a.K, vel, ux, uy, uz, rho) + feq_q(a.K, vel, ux, uy, uz, R(1));
-    }
+    interp_outlet(d, b, a.pub_post, a.K, a.cs, a.crev20 != 0);
     store_node(a, d, pl, ix);
 }
 
@@ -1050,6 +1114,11 @@ struct AuxArgs {
     double* io;     // staging buffer (FP64, caller layout)
     int sp_axis, sp_begin;      // outlet omega damping (as StepArgs)
     typename ST::R sp_deriv;
+    int outl_pending;           // deferred outlet: CL_OUTL cells still need the rule
+    const NodeDev* outl;
+    const typename ST::R* pub_prev;   // records of the step that produced the state
+    typename ST::R cs;
+    int crev20;
 };
 
 // Thread -> caller cell -> padded cell.
@@ -1067,7 +1136,8 @@ __device__ __forceinline__ bool user_cell(const Geo& g, uint32_t& iu, Cell& c)
     return true;
 }
 
-// Current populations of a non-solid cell.
+// Current populations of a non-solid cell (a deferred outlet cell's with its
+// pending rule applied).
 template <class ST>
 __device__ __forceinline__ void cell_state(const AuxArgs<ST>& a, const Cell& c, uint8_t cls, typename ST::R* f)
 {
@@ -1075,11 +1145,14 @@ __device__ __forceinline__ void cell_state(const AuxArgs<ST>& a, const Cell& c, 
         const int bi = a.bcidx[c.i];
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
-    } else if (a.odd) {
-        load_pop<ST, true>(a.p, a.rest, c, f);
-    } else {
-        load_pop<ST, false>(a.p, a.rest, c, f);
+        return;
     }
+    if (a.odd)
+        load_pop<ST, true>(a.p, a.rest, c, f);
+    else
+        load_pop<ST, false>(a.p, a.rest, c, f);
+    if (cls == CL_OUTL && a.outl_pending)
+        deferred_outlet<ST>(f, a.outl[a.bcidx[c.i]], a.pub_prev, a.K, a.cs, a.crev20 != 0, a.rest);
 }
 
 // f = f_eq(rho, u) from the macro planes (InitializationEquilibriumConstVector.h:111-126);
@@ -1140,6 +1213,12 @@ __global__ __launch_bounds__(TPB) void k_gather(AuxArgs<ST> a)
     const int q = a.q;
     if (cls == CL_INLET) {
         a.io[iu] = (double)a.inl_state[(size_t)a.bcidx[c.i] * Q + q];
+        return;
+    }
+    if (cls == CL_OUTL && a.outl_pending) {
+        typename ST::R f[Q];
+        cell_state(a, c, cls, f);
+        a.io[iu] = (double)f[q];
         return;
     }
     int pl;

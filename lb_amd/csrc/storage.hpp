@@ -60,6 +60,8 @@ struct StoreF64 {
     using R = double;   // arithmetic type
     static constexpr bool dev = false;
     static constexpr bool sr = false;
+    // the value a store then a load would return
+    __device__ __forceinline__ static R round_trip(R v, R) { return v; }
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
     template <bool NT>
@@ -79,6 +81,7 @@ struct StoreF32 {
     using R = float;
     static constexpr bool dev = false;
     static constexpr bool sr = false;
+    __device__ __forceinline__ static R round_trip(R v, R) { return v; }
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R) { return at(p, i); }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R) { at(p, i) = v; }
     template <bool NT>
@@ -100,6 +103,7 @@ struct StoreF16Dev {
     static constexpr bool sr = false;
     __device__ __forceinline__ static R load(const T* p, uint32_t i, R rest) { return (float)at(p, i) + rest; }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = (_Float16)(v - rest); }
+    __device__ __forceinline__ static R round_trip(R v, R rest) { return (float)(_Float16)(v - rest) + rest; }
     template <bool NT>
     __device__ __forceinline__ static R get(const T* p, uint32_t i, R rest)
     {
@@ -134,6 +138,10 @@ struct StoreBF16Dev {
         return __builtin_bit_cast(uint16_t, h);
     }
     __device__ __forceinline__ static void store(T* p, uint32_t i, R v, R rest) { at(p, i) = rne(v, rest); }
+    __device__ __forceinline__ static R round_trip(R v, R rest)
+    {
+        return __uint_as_float((uint32_t)rne(v, rest) << 16) + rest;
+    }
     template <bool NT>
     __device__ __forceinline__ static void put(T* p, uint32_t i, R v, R rest, uint32_t)
     {
