@@ -26,7 +26,11 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are disabled for every HIP translation unit.
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # matrix-core results in ordinary VGPRs: the closure kernels read
+             # every result with VALU instructions, and accumulators in AGPRs
+             # cost one v_accvgpr_read per value (128 per wave in the step)
+             "-mllvm", "-amdgpu-mfma-vgpr-form=true"]
 CXX = shutil.which("g++") or "g++"
 CC = shutil.which("gcc") or "gcc"
 
@@ -37,8 +41,10 @@ def _run(cmd, cwd=None):
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"build step failed: {cmd[0]} ... (exit {r.returncode})")
-    # hipcc's host pass does not know the packed-fp32 feature; that warning is harmless
-    noise = [ln for ln in (r.stderr or "").splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
+    # hipcc's host pass does not know the packed-fp32 feature, and the link
+    # step does not use the code-generation option; those warnings are harmless
+    noise = [ln for ln in (r.stderr or "").splitlines()
+             if "packed-fp32-ops" not in ln and "amdgpu-mfma-vgpr-form" not in ln and ln.strip()]
     if noise:
         sys.stderr.write("\n".join(noise) + "\n")
 

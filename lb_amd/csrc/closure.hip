@@ -644,6 +644,7 @@ lbm_status lbm_mlp_forward(lbm_mlp* m, const float* x, int64_t n, float c_les, f
     W.b1 = m->p + OFF_B1;
     W.b2 = m->p + OFF_B2;
     W.w3 = m->p + OFF_W3;
+    W.norm = nullptr;   // the caller passes normalised features
     W.b3 = p[OFF_B3];
     W.c = c_les;
     if (hipMemcpy(dx, x, sizeof(float) * NFEAT * n, hipMemcpyHostToDevice) != hipSuccess) rs = LBM_E_HIP;

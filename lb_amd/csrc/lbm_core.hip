@@ -67,7 +67,7 @@ struct lbm_ctx {
     float* cl_params = nullptr;   // FP32 parameters
     uint16_t* cl_wb = nullptr;    // BF16 W1 [64][16] + W2 [64][64]
     float cl_b3 = 0.0f;
-    MlpNorm cl_norm{};
+    float* cl_norm = nullptr;     // [4][12] mean, 1/sd, envelope low, high
     float* ccache = nullptr;      // coefficient per cell held between network evaluations
     int cl_every = 1;             // evaluate the network every cl_every steps
     int64_t cl_count = 0;         // steps since the last (re)start of the refresh cycle
@@ -159,6 +159,7 @@ static MlpWeights closure_weights(const lbm_ctx* c)
     W.b1 = c->cl_params ? c->cl_params + OFF_B1 : nullptr;
     W.b2 = c->cl_params ? c->cl_params + OFF_B2 : nullptr;
     W.w3 = c->cl_params ? c->cl_params + OFF_W3 : nullptr;
+    W.norm = c->cl_norm;
     W.b3 = c->cl_b3;
     W.c = (float)c->d.c_les;
     return W;
@@ -313,7 +314,6 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
     if (mlp) {
         if (!c->mlp_ready) return LBM_E_STATE;
         ma.mlp = closure_weights(c);
-        ma.norm = c->cl_norm;
     }
     const bool patches = c->n_upd || c->n_eq || c->n_outlet || c->n_ghost;
     const dim3 grid((unsigned)c->g.ntiles), block(TPB);
@@ -840,7 +840,7 @@ void lbm_destroy(lbm_ctx* c)
 {
     if (!c) return;
     void* bufs[] = {c->pops,     c->cls,      c->node,  c->bcidx, c->inl_state, c->inl,      c->outl,
-                    c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params,
+                    c->ghost,    c->pub_mac,  c->pub_post, c->macro, c->stage, c->partials, c->cl_params, c->cl_norm,
                     c->cl_wb,    c->ccache,   c->avg,   c->probe_buf, c->inl_post, c->inl_mac,
                     c->flag1,    c->uerr,     c->snap_dev, c->pub_post2};
     for (void* b : bufs)
@@ -1224,7 +1224,6 @@ static lbm_status run_closure_eval(lbm_ctx* c, float* feats, int32_t* env, doubl
             std::memset(&e, 0, sizeof(e));
             e.a = aux_args<ST>(c);
             e.mlp = closure_weights(c);
-            e.norm = c->cl_norm;
             e.feats = feats;
             e.env = env;
             e.coef = coef;
@@ -1671,14 +1670,14 @@ lbm_status lbm_set_closure(lbm_ctx* c, const float* params, const float* norm)
 {
     if (!c || !params || !norm) return LBM_E_ARG;
     if (c->d.closure != LBM_CLOSURE_MLP) return LBM_E_STATE;
-    MlpNorm N;
+    float N[4 * NFEAT];   // mean, 1/sd, envelope low, high
     for (int k = 0; k < NFEAT; ++k) {
         const float sd = norm[NFEAT + k];
-        N.mu[k] = norm[k];
-        N.isd[k] = 1.0f / sd;
-        N.lo[k] = norm[2 * NFEAT + k];
-        N.hi[k] = norm[3 * NFEAT + k];
-        if (!(sd > 0.0f) || !std::isfinite(sd) || !std::isfinite(N.mu[k]) || !std::isfinite(N.isd[k]))
+        N[k] = norm[k];
+        N[NFEAT + k] = 1.0f / sd;
+        N[2 * NFEAT + k] = norm[2 * NFEAT + k];
+        N[3 * NFEAT + k] = norm[3 * NFEAT + k];
+        if (!(sd > 0.0f) || !std::isfinite(sd) || !std::isfinite(N[k]) || !std::isfinite(N[NFEAT + k]))
             return LBM_E_ARG;
     }
     for (int i = 0; i < NPARAM; ++i)
@@ -1691,17 +1690,21 @@ lbm_status lbm_set_closure(lbm_ctx* c, const float* params, const float* norm)
         lbm_status st = dev_alloc(c, (void**)&c->cl_wb, sizeof(uint16_t) * (NHID * 16 + NHID * NHID));
         if (st != LBM_OK) return st;
     }
+    if (!c->cl_norm) {
+        lbm_status st = dev_alloc(c, (void**)&c->cl_norm, sizeof(N));
+        if (st != LBM_OK) return st;
+    }
     std::vector<uint16_t> wb(NHID * 16 + NHID * NHID);
     lbm_mlp_pack_bf16(params, wb.data(), wb.data() + NHID * 16);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->cl_params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->cl_wb, wb.data(), sizeof(uint16_t) * wb.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->cl_norm, N, sizeof(N), hipMemcpyHostToDevice));
     if (!c->ccache) {
         lbm_status st = dev_alloc(c, (void**)&c->ccache, sizeof(float) * c->N);
         if (st != LBM_OK) return st;
     }
     c->cl_b3 = params[OFF_B3];
-    c->cl_norm = N;
     c->cl_count = 0;
     c->mlp_ready = true;
     return LBM_OK;

@@ -212,7 +212,6 @@ template <class ST>
 struct MlpStepArgs {
     StepArgs<ST> s;
     MlpWeights mlp;
-    MlpNorm norm;
 };
 
 struct Cell {
@@ -758,7 +757,7 @@ struct ClosurePrep {
 template <class ST>
 __device__ __forceinline__ void closure_prep(const typename ST::R* f, bool first, const typename ST::R* macro,
                                              size_t N, uint32_t i, typename ST::R tau, typename ST::R cles,
-                                             const MlpNorm& norm, ClosurePrep<ST>& o)
+                                             const MlpLds& L, ClosurePrep<ST>& o)
 {
     using R = typename ST::R;
     const Raw2<R> r = raw_moments(f);
@@ -779,7 +778,7 @@ __device__ __forceinline__ void closure_prep(const typename ST::R* f, bool first
     o.om_s = omega_les_fast(o.ce, o.rho, tau, cles);
     R om_unused;
     cell_features(o.ce, o.rho, tau, cles, o.x, o.pin, om_unused);
-    o.env = mlp_normalise(norm, o.x, o.xn);
+    o.env = mlp_normalise(L, o.x, o.xn);
 }
 
 // Collide-stream step with the learned closure (fast arithmetic, FP32 math).
@@ -819,7 +818,7 @@ __global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
         } else {
             load_pop<ST, ODD>(a.p, a.rest, c, f);
         }
-        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, ma.norm, cp);
+        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, L, cp);
         if (cp.env) {
 #pragma unroll
             for (int k = 0; k < NFEAT; ++k) xn[k] = cp.xn[k];
@@ -1460,7 +1459,6 @@ template <class ST>
 struct ClosureEvalArgs {
     AuxArgs<ST> a;
     MlpWeights mlp;
-    MlpNorm norm;
     float* feats;     // [12][NU] normalised features, or null
     int32_t* env;     // [NU], or null
     double* coef;     // [NU] coefficient the next step uses, or null
@@ -1490,8 +1488,7 @@ __global__ __launch_bounds__(TPB) void k_closure_eval(ClosureEvalArgs<ST> e)
     if (active) {
         R f[Q];
         cell_state(a, c, a.cls[c.i], f);
-        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, e.norm,
-                         cp);
+        closure_prep<ST>(f, a.first != 0, a.macro, (size_t)a.g.nx * a.g.ny * a.g.nz, c.i, a.tau, a.cles, L, cp);
         if (cp.env) {
 #pragma unroll
             for (int k = 0; k < NFEAT; ++k) xn[k] = cp.xn[k];

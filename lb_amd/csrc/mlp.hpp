@@ -72,21 +72,21 @@ struct MlpWeights {
     const float* b1;      // [64]
     const float* b2;      // [64]
     const float* w3;      // [64]
+    const float* norm;    // [4][12]: mean, 1/standard deviation, envelope low, high (or null)
     float b3;
     float c;              // head scale (Smagorinsky constant)
 };
 
-// Input normalisation and training envelope (normalised units); isd holds
-// the reciprocal standard deviations, so normalising costs no division.
-struct MlpNorm {
-    float mu[NFEAT], isd[NFEAT], lo[NFEAT], hi[NFEAT];
-};
-
-// LDS image of the weights plus a per-wave feature staging area.
+// LDS image of the weights and the input normalisation, plus a per-wave
+// feature staging area.  The normalisation sits here rather than in kernel
+// arguments: its 48 values held in scalar registers through the step made
+// the compiler spill scalar registers.
 struct MlpLds {
     uint16_t w1[NHID * 16];
     uint16_t w2[NHID * NHID];
     float b1[NHID], b2[NHID], w3[NHID];
+    float norm[4][NFEAT];      // mean, 1/sd (no division to normalise), envelope low, high
+                               // (the envelope in normalised units)
     uint16_t xs[4][64 * 16];   // per wave: [lane][feature], BF16
 };
 
@@ -106,6 +106,7 @@ __device__ __forceinline__ void mlp_load_lds(MlpLds& L, const MlpWeights& W)
         L.b2[t] = W.b2[t];
         L.w3[t] = W.w3[t];
     }
+    if (W.norm && t < 4 * NFEAT) (&L.norm[0][0])[t] = W.norm[t];
 }
 
 // Ordering of LDS accesses between lanes of one wave.
@@ -165,11 +166,14 @@ __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, fl
         f32x4 bias;
 #pragma unroll
         for (int j = 0; j < 4; ++j) bias[j] = L.b1[hb * 16 + 4 * q + j];
+        // the four products first, then their activations: the matrix core
+        // works on the next product while the activations run
+        f32x4 c[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const f32x4 c = mfma_bf16(w, xb[t], bias);
-            a1[t][hb] = pack_bf16x4(silu_hw(c[0]), silu_hw(c[1]), silu_hw(c[2]), silu_hw(c[3]));
-        }
+        for (int t = 0; t < 4; ++t) c[t] = mfma_bf16(w, xb[t], bias);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            a1[t][hb] = pack_bf16x4(silu_hw(c[t][0]), silu_hw(c[t][1]), silu_hw(c[t][2]), silu_hw(c[t][3]));
     }
     float zp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -213,16 +217,20 @@ __device__ __forceinline__ float mlp_wave_forward(MlpLds& L, const float* xn, fl
     return zme + b3;
 }
 
-// Normalise features; true when every one lies inside the training envelope.
-__device__ __forceinline__ bool mlp_normalise(const MlpNorm& N, const float* x, float* xn)
+// Normalise features with the LDS image's mean and 1/sd; true when every
+// one lies inside the training envelope.  The tests combine without
+// short-circuit: a chain of per-feature branches made the compiler shuffle
+// (and spill) scalar registers at every one.
+__device__ __forceinline__ bool mlp_normalise(const MlpLds& L, const float* x, float* xn)
 {
-    bool ok = true;
+    unsigned ok = 1u;
 #pragma unroll
     for (int k = 0; k < NFEAT; ++k) {
-        xn[k] = (x[k] - N.mu[k]) * N.isd[k];
-        ok = ok && xn[k] >= N.lo[k] && xn[k] <= N.hi[k];
+        xn[k] = (x[k] - L.norm[0][k]) * L.norm[1][k];
+        // inside [lo, hi] exactly when the median of (lo, x, hi) is x (false for NaN)
+        ok &= (unsigned)(__builtin_amdgcn_fmed3f(L.norm[2][k], xn[k], L.norm[3][k]) == xn[k]);
     }
-    return ok;
+    return ok != 0u;
 }
 
 __device__ __forceinline__ float mlp_coefficient(float z, float c) { return c * softplus_f(z) / LN2F; }
