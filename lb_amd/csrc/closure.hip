@@ -374,11 +374,13 @@ uint16_t host_bf16(float x)
 
 }  // namespace
 
-// BF16 inference image of a parameter vector: W1 padded to [64][16], W2.
+// BF16 inference image of a parameter vector: -log2(e) W1 padded to
+// [64][16] (inference works in the scaled variable g = -log2(e) h), W2.
 void lbm_mlp_pack_bf16(const float* p, uint16_t* w1, uint16_t* w2)
 {
     for (int h = 0; h < NHID; ++h)
-        for (int k = 0; k < 16; ++k) w1[h * 16 + k] = k < NFEAT ? host_bf16(p[OFF_W1 + h * NFEAT + k]) : 0;
+        for (int k = 0; k < 16; ++k)
+            w1[h * 16 + k] = k < NFEAT ? host_bf16(NEG_LOG2E * p[OFF_W1 + h * NFEAT + k]) : 0;
     for (int i = 0; i < NHID * NHID; ++i) w2[i] = host_bf16(p[OFF_W2 + i]);
 }
 

@@ -87,6 +87,18 @@ LBM_HD R frcp(R x)
     return R(1) / x;
 }
 
+// Base-10 logarithm for the closure features: the hardware log2 (within
+// 1 ulp for normal inputs) times log10(2) on the device in FP32, the library
+// function otherwise.
+template <class R>
+LBM_HD R log10_r(R x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_logf(x) * 0.30102999566398120f;
+#endif
+    return std::log10(x);
+}
+
 // ---------------------------------------------------------------------------
 // Equilibrium and moments, literal operation order.
 // ---------------------------------------------------------------------------

@@ -33,7 +33,7 @@ LBM_HD void strain_features(R sxx, R syy, R szz, R sxy, R sxz, R syz, R rho, R t
     const R n2 = sxx * sxx + syy * syy + szz * szz + R(2) * (sxy * sxy + sxz * sxz + syz * syz);
     const R n = sqrt_r(n2);
     const R inv = n > R(0) ? R(1) / n : R(0);
-    x[0] = n > R(0) ? (R)std::log10(n) : R(-30);
+    x[0] = n > R(0) ? log10_r(n) : R(-30);
     x[1] = sxx * inv;
     x[2] = syy * inv;
     x[3] = szz * inv;
@@ -50,7 +50,7 @@ LBM_HD void strain_features(R sxx, R syy, R szz, R sxy, R sxz, R syz, R rho, R t
                  R(3) * dyy * (sxy * sxy + syz * syz) + R(3) * dzz * (sxz * sxz + syz * syz) + R(6) * sxy * sxz * syz;
     x[8] = dn > R(0) ? R(2.449489742783178) * d3 / (dn * dn * dn) : R(0);
     x[9] = R(100) * (rho - R(1));
-    x[10] = n > R(0) ? (R)std::log10(n / ((tau0 - R(0.5)) * R(1.0 / 3.0))) : R(-30);
+    x[10] = n > R(0) ? x[0] - log10_r((tau0 - R(0.5)) * R(1.0 / 3.0)) : R(-30);
     x[11] = dn * inv;
 }
 

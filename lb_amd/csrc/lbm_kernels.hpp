@@ -707,15 +707,22 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             }
         }
         // streaming writes, shared by every lane of the wave (boundary lanes drop
-        // the writes their mask skips)
+        // the writes their mask skips).  On the odd step their 27 neighbour
+        // offsets are those of the loads; formed once and kept through the
+        // collision they held 13-20 more registers (one wave per SIMD less),
+        // so they are formed again from an opaque copy of the cell.
+        Cell cst = c;
+        if constexpr (ODD)
+            __asm__ volatile("" : "+v"(cst.i), "+v"(cst.dxm), "+v"(cst.dxp), "+v"(cst.dym), "+v"(cst.dyp),
+                             "+v"(cst.dzm), "+v"(cst.dzp));
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, true>(a.p, a.rest, c, q, P[q], hsr);
+                store_out<ST, ODD, true>(a.p, a.rest, cst, q, P[q], hsr);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
+                store_out<ST, ODD, false>(a.p, a.rest, cst, q, P[q], hsr);
     }
 
     if (a.diag) {
@@ -836,7 +843,13 @@ __global__ __launch_bounds__(TPB) void k_step_mlp(MlpStepArgs<ST> ma)
             for (int q = 0; q < Q; ++q) P[q] = f[q];
             srt_fast(a.K, P, cp.rho, cp.ux, cp.uy, cp.uz, omega);
         }
-        stream_cell<ST, ODD, FLAGS>(a, c, cls, bi, inlet, P, cp.rho, cp.ux, cp.uy, cp.uz);
+        // the streaming offsets are formed again here: kept from the loads of
+        // an odd step, the 27 neighbour offsets held 24 registers through the
+        // network
+        Cell cs = c;
+        __asm__ volatile("" : "+v"(cs.i), "+v"(cs.dxm), "+v"(cs.dxp), "+v"(cs.dym), "+v"(cs.dyp), "+v"(cs.dzm),
+                         "+v"(cs.dzp));
+        stream_cell<ST, ODD, FLAGS>(a, cs, cls, bi, inlet, P, cp.rho, cp.ux, cp.uy, cp.uz);
     }
 }
 

@@ -58,17 +58,35 @@ def softplus(z):
 
 
 def forward(p, X, c, round_bf16=False):
-    """Coefficient C for rows of X (n, 12); round_bf16 rounds weights and
-    layer inputs to BF16 as the matrix cores see them."""
+    """Coefficient C for rows of X (n, 12).  round_bf16 evaluates the network
+    as the matrix-core inference does: in the scaled variable g = -log2(e) h
+    (logistic 1 / (1 + 2^g)), with the FP32 products -log2(e) W1 rounded to
+    BF16, FP32 -log2(e) b1, -log2(e) b2 and -ln(2) w3, and the inputs and
+    the scaled layer-1 activations rounded to BF16.  Without rounding the
+    scaling cancels and this is the plain network."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
     X = np.asarray(X, dtype=np.float64)
-    r = bf16 if round_bf16 else (lambda a: a)
-    H1 = r(X) @ r(W1).T + b1
-    A1 = silu(H1)
-    H2 = r(A1) @ r(W2).T + b2
-    A2 = silu(H2)
-    z = A2 @ w3 + b3
-    return c * softplus(z) / LN2, (X, H1, A1, H2, A2, z)
+    if not round_bf16:
+        H1 = X @ W1.T + b1
+        A1 = silu(H1)
+        H2 = A1 @ W2.T + b2
+        A2 = silu(H2)
+        z = A2 @ w3 + b3
+        return c * softplus(z) / LN2, (X, H1, A1, H2, A2, z)
+    f32 = np.float32
+    k2, kn = f32(-1.4426950408889634), f32(-0.69314718055994531)
+
+    def scaled(k, a):   # FP32 product k * a
+        return (k * np.asarray(a, dtype=f32)).astype(np.float64)
+
+    def gsilu(g):       # g / (1 + 2^g) = -log2(e) SiLU(h)
+        return g / (1.0 + np.exp2(g))
+
+    G1 = bf16(X) @ bf16(scaled(k2, W1)).T + scaled(k2, b1)
+    A1 = bf16(gsilu(G1))
+    G2 = A1 @ bf16(W2).T + scaled(k2, b2)
+    z = gsilu(G2) @ scaled(kn, w3) + b3
+    return c * softplus(z) / LN2, (X, G1, A1, G2, gsilu(G2), z)
 
 
 def loss_grad(p, X, y, w, c):
