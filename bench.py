@@ -216,6 +216,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip the other storage formats")
     ap.add_argument("--no-c1", action="store_true", help="skip the C1 cavity row")
+    ap.add_argument("--closure-only", action="store_true",
+                    help="time only the plain FP32 step and the learned-closure step (network every step), "
+                         "print one line each and stop (for profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -247,6 +250,11 @@ def main():
         f"wall {res['ms_per_step']:.4f} ms/step, mass {res['mass']:.6e}")
     if not np.isfinite(res["mass"]):
         raise SystemExit("non-finite mass after the timed run")
+    if args.closure_only:
+        r = run_gpu(n, abi.STORAGE_F32, fields, args.steps, args.warmup, mlp=True, refresh=1)
+        log(f"f32 + learned closure (every 1): kernel {r['kernel_ms']:.4f} ms/step "
+            f"({r['kernel_ms'] / res['kernel_ms'] - 1:+.1%} over the plain step)")
+        return
 
     cells = n ** 3
     bytes_per_launch = cells * 2 * 27 * sbytes
