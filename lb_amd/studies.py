@@ -11,11 +11,19 @@ c4  flow past a voxelized obstacle, 512 x 256 x 256: synthetic OFF scene,
     cumulant + Smagorinsky; drag from the wall-link momentum exchange, FP32
     and FP16-deviation storage (stochastic and nearest rounding) against FP64
     storage, MLUPS.
+snapshot  output overlap of the host driver: a lid-driven cavity (cumulant +
+    Smagorinsky, FP32) run with VTK output every `every` steps, with the
+    synchronous read-back and with the asynchronous snapshot path
+    (async_output), against the same run without output; wall times, the
+    share of the stepping time that the asynchronous path hides behind the
+    file writes, and a check that both write identical files.
 
-usage: python -m lb_amd.studies c3|c4 [--steps N] [--out file.json]
+usage: python -m lb_amd.studies c3|c4|snapshot [--steps N] [--out file.json]
 """
 from __future__ import annotations
 
+import ctypes
+import hashlib
 import json
 import os
 import tempfile
@@ -178,11 +186,77 @@ def c4(steps=5000, every=250, fp64_steps=None):
     return res
 
 
+def _cavity_case(Lh, n, steps, results_dir, **extra):
+    """The host driver's lid-driven cavity (n^3 fluid cells, moving-wall lid),
+    as the reference's case files set one up."""
+    h = ctypes.c_void_p()
+    if Lh.lbmh_case_create(ctypes.byref(h)) != 0:
+        raise RuntimeError("lbmh_case_create failed")
+    grid = np.zeros((n + 2,) * 3, np.int32)
+    grid[1:-1, 1:-1, 1:-1] = 1
+    dims = (ctypes.c_int32 * 3)(n + 2, n + 2, n + 2)
+    Lh.lbmh_case_set_grid(h, dims, grid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 1.0, 1.0,
+                          (ctypes.c_float * 3)(0, 0, 0), (ctypes.c_float * 3)(n, n, n))
+    tau = 0.56
+    params = dict(tau=tau, ny=(tau - 0.5) / 3, rho_fyz=1.0, Cm=1.0, Cl=1.0, Cu_inverse=1.0, iterations=steps,
+                  err_every_it=-1, plot_every_it=-1, iterationsMomentAvg=0, collision=abi.COLLISION_CUM2015,
+                  closure=abi.CLOSURE_SMAGORINSKY, CLES=0.17, storage=abi.STORAGE_F32, arith=abi.ARITH_FAST,
+                  compat=abi.COMPAT_REFERENCE)
+    params.update(extra)
+    for k, v in params.items():
+        if Lh.lbmh_case_set(h, k.encode(), float(v)) != 0:
+            raise RuntimeError(f"lbmh_case_set {k}")
+    os.makedirs(results_dir, exist_ok=True)
+    Lh.lbmh_case_set_results_dir(h, (str(results_dir) + "/").encode())
+    ok = Lh.lbmh_case_mesh_wall(h) == 0
+    lid = (ctypes.c_float * 3)(0.05, 0.0, 0.0)
+    ok = ok and Lh.lbmh_case_inlet(h, (ctypes.c_double * 3)(-1.0, n - 0.75, -1.0),
+                                   (ctypes.c_double * 3)(n + 1.0, n - 0.25, n + 1.0),
+                                   (ctypes.c_double * 3)(-1.0, n - 0.75, n + 1.0), -1,
+                                   (ctypes.c_float * 3)(0, 1, 0), lid, abi.BC_INLET_MOVING_WALL) == 0
+    ok = ok and Lh.lbmh_case_compile(h) == 0 and Lh.lbmh_case_initialize(h) == 0
+    if not ok:
+        raise RuntimeError("cavity case set-up failed")
+    return h
+
+
+def snapshot(n=128, every=2000, plots=5):
+    Lh = host.load_host()
+    steps = every * plots
+    res = {"workload": f"lid-driven cavity {n}^3 (moving-wall lid u=0.05), cumulant-2015 + Smagorinsky, FP32, "
+                       f"{steps} steps, VTK 'variables' output every {every} steps", "cells": n ** 3}
+    digests = {}
+    with tempfile.TemporaryDirectory() as td:
+        for name, extra in (("no_output", {}), ("sync", dict(plot_every_it=every, async_output=0)),
+                            ("async", dict(plot_every_it=every, async_output=1))):
+            d = os.path.join(td, name)
+            h = _cavity_case(Lh, n, steps, d, **extra)
+            t0 = time.perf_counter()
+            rc = Lh.lbmh_case_run(h)
+            wall = time.perf_counter() - t0
+            Lh.lbmh_case_destroy(h)
+            if rc != 0:
+                raise RuntimeError(f"{name}: lbmh_case_run returned {rc}")
+            files = sorted(f for f in os.listdir(d) if f.startswith("variables."))
+            digests[name] = {f: hashlib.sha256(open(os.path.join(d, f), "rb").read()).hexdigest() for f in files}
+            size = sum(os.path.getsize(os.path.join(d, f)) for f in files)
+            res[name] = {"wall_s": wall, "files": len(files), "bytes_written": size}
+            _log(f"{name}: {wall:.2f} s, {len(files)} files, {size / 1e6:.0f} MB")
+            for f in files:
+                os.remove(os.path.join(d, f))
+    res["files_identical"] = digests["sync"] == digests["async"] and len(digests["sync"]) > 0
+    t0, ts, ta = res["no_output"]["wall_s"], res["sync"]["wall_s"], res["async"]["wall_s"]
+    res["stepping_hidden_fraction"] = (ts - ta) / t0
+    _log(f"files identical: {res['files_identical']}; stepping time hidden behind output: "
+         f"{res['stepping_hidden_fraction']:.0%}")
+    return res
+
+
 if __name__ == "__main__":
     import argparse
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["c3", "c4"])
+    ap.add_argument("study", choices=["c3", "c4", "snapshot"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--n", type=int, default=384)
     ap.add_argument("--every", type=int, default=1000)
@@ -192,6 +266,8 @@ if __name__ == "__main__":
     if a.study == "c3":
         kw = {"storages": tuple(a.storages.split(","))} if a.storages else {}
         r = c3(n=a.n, steps=a.steps or 10000, every=a.every, **kw)
+    elif a.study == "snapshot":
+        r = snapshot(n=a.n if a.n != 384 else 128, every=a.every if a.every != 1000 else 2000)
     else:
         r = c4(steps=a.steps or 5000)
     if a.out:

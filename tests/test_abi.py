@@ -94,10 +94,14 @@ def test_validation_rejects_bad_descriptors():
 def test_largest_lattice_limit(storage, n_over):
     """A population plane is addressed with 32-bit byte offsets: lattices whose
     padded plane reaches 4 GiB (or 2^31 cells) are refused at create, before
-    the flag grid is read (a one-cell flag buffer stands in for it here)."""
-    flags = np.ones((1, 1, 1), np.int32)
+    the flag grid is read.  The flag buffer has the full size (zero pages the
+    kernel maps lazily, so it costs no memory), so that a create which read it
+    after all would still read inside it and fail this test instead of
+    reading past the end."""
+    nx, ny, nz = n_over
+    flags = np.zeros((nz, ny, nx), np.int32)
     d = make_desc(flags, [], tau=0.6, storage=storage, arith=abi.ARITH_FAST, periodic=(1, 1, 1))
-    d.nx, d.ny, d.nz = n_over
+    assert (d.nx, d.ny, d.nz) == n_over
     assert _create_status(d) == abi.LBM_E_ARG
 
 
