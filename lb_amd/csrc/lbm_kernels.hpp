@@ -123,6 +123,7 @@ struct StepArgs {
     typename ST::R sp_deriv;       // omega decrement per cell past sp_begin
     int outl_pending;              // CL_OUTL cells' loaded state still needs the outlet rule
     typename ST::R cs;             // the outlet's interpolation weight
+    int inl_pending;               // inlet cells' stored state still needs its rule (deferred inlet)
 };
 
 // Outlet omega damping (OutletNeighbourEquilibriumOmega.h:18-79) of a cell at
@@ -417,6 +418,19 @@ __device__ __forceinline__ void zou_he(const Consts<R>& K, const NodeDev& b, boo
     }
 }
 
+// The update a moving-wall or Zou-He inlet cell's rule makes to its stored
+// state st after a step, from that step's records: the cell's own
+// post-collision populations (post) and its rho, u (own).
+template <class R>
+__device__ __forceinline__ void inlet_rule(const Consts<R>& K, const NodeDev& b, const R* post, const R* own,
+                                           bool crev20, R* st)
+{
+    if (b.kind == LBM_BC_INLET_MOVING_WALL)
+        moving_wall(K, b, crev20, post, own[0], st);
+    else if (b.kind == LBM_BC_INLET_ZOU_HE)
+        zou_he(K, b, crev20, post, own[0], own[1], own[2], own[3], st);
+}
+
 template <class R>
 __device__ __forceinline__ R fast_rcp(R x)
 {
@@ -511,7 +525,10 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
             store_out<ST, ODD, false>(a.p, a.rest, c, q, P[q], hsr);
 }
 
-template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false>
+// INL: inlet rules deferred into this kernel (a separate instantiation: the
+// rule's code, even untaken, cost the obstacle workload's flagged step 1-2 %
+// in FP32/FP64 and 8 % in FP16 storage).
+template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false, bool INL = false>
 __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 {
     using R = typename ST::R;
@@ -534,9 +551,23 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
         int bi = -1;
         const bool inlet = FLAGS && cls == CL_INLET;
         if (FLAGS && cls >= CL_INLET) bi = a.bcidx[c.i];
-        if (inlet) {
+        if (inlet && !INL) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
+        } else if (inlet) {
+            R* stp = a.inl_state + (size_t)bi * Q;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = stp[q];
+            if (a.inl_pending) {
+                // deferred inlet: the moving-wall or Zou-He rule of the
+                // previous step, from that step's records, applied to the
+                // stored state and stored (this cell streams it below, and
+                // read-backs see it)
+                inlet_rule(a.K, a.inl[bi], a.inl_post + (size_t)bi * Q, a.inl_mac + (size_t)bi * PUB_MAC,
+                           a.crev20 != 0, f);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) stp[q] = f[q];
+            }
         } else {
             if (!FLAGS) load_pop<ST, ODD>(a.p, a.rest, c, f);
             // deferred interpolated outlet (4- and 8-byte storage only): the
@@ -1138,6 +1169,10 @@ struct AuxArgs {
     const typename ST::R* pub_prev;   // records of the step that produced the state
     typename ST::R cs;
     int crev20;
+    int inl_pending;            // deferred inlet: inlet cells' stored state still needs its rule
+    const NodeDev* inl;
+    const typename ST::R* inl_post;
+    const typename ST::R* inl_mac;
 };
 
 // Thread -> caller cell -> padded cell.
@@ -1164,6 +1199,9 @@ __device__ __forceinline__ void cell_state(const AuxArgs<ST>& a, const Cell& c, 
         const int bi = a.bcidx[c.i];
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
+        if (a.inl_pending)
+            inlet_rule(a.K, a.inl[bi], a.inl_post + (size_t)bi * Q, a.inl_mac + (size_t)bi * PUB_MAC,
+                       a.crev20 != 0, f);
         return;
     }
     if (a.odd)
@@ -1230,14 +1268,14 @@ __global__ __launch_bounds__(TPB) void k_gather(AuxArgs<ST> a)
         return;
     }
     const int q = a.q;
-    if (cls == CL_INLET) {
-        a.io[iu] = (double)a.inl_state[(size_t)a.bcidx[c.i] * Q + q];
-        return;
-    }
-    if (cls == CL_OUTL && a.outl_pending) {
+    if ((cls == CL_INLET && a.inl_pending) || (cls == CL_OUTL && a.outl_pending)) {
         typename ST::R f[Q];
         cell_state(a, c, cls, f);
         a.io[iu] = (double)f[q];
+        return;
+    }
+    if (cls == CL_INLET) {
+        a.io[iu] = (double)a.inl_state[(size_t)a.bcidx[c.i] * Q + q];
         return;
     }
     int pl;
