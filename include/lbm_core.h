@@ -93,9 +93,14 @@ enum { LBM_ARITH_LITERAL = 0, LBM_ARITH_FAST = 1 };
                                       D_x u + D_y v - 2 D_z w; the axis convention of Eq. 43 (and
                                       symmetry under x <-> z) gives -2 D_x u + D_y v + D_z w.  The
                                       reference's form exchanges the A terms of C_220 and C_022 */
+#define LBM_COMPAT_OMEGA4    0x40u /* Cum2017.h:259-260, Combined.h:259-260 omega_4's denominator
+                                      omega_2 (56 - 42 omega_1 + 8 omega_1^2) - 8 omega_1, where
+                                      the published rate (Geier, Pasquali, Schoenherr 2017) has
+                                      9 omega_1^2: with 8 the 2017 operator's viscosity is
+                                      second-order accurate, with 9 fourth-order */
 #define LBM_COMPAT_REFERENCE                                                                       \
     (LBM_COMPAT_CREV20 | LBM_COMPAT_F32CONST | LBM_COMPAT_LIM210 | LBM_COMPAT_TRACE17 | LBM_COMPAT_INLETRHO | \
-     LBM_COMPAT_EQ44)
+     LBM_COMPAT_EQ44 | LBM_COMPAT_OMEGA4)
 
 /* Boundary-node kinds.  The flag grid keeps the reference encoding
  * (geometryMesherBoundary.h): 0 solid, >0 fluid (1 + number of solid

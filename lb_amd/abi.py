@@ -25,9 +25,9 @@ STORAGE_F64, STORAGE_F32, STORAGE_F16DEV, STORAGE_BF16DEV = 0, 1, 2, 3
 STORAGE_F16DEV_SR, STORAGE_BF16DEV_SR = 4, 5   # stochastic rounding in the step's stores
 ARITH_LITERAL, ARITH_FAST = 0, 1
 COMPAT_CREV20, COMPAT_F32CONST, COMPAT_LIM210, COMPAT_TRACE17 = 0x1, 0x2, 0x4, 0x8
-COMPAT_INLETRHO, COMPAT_EQ44 = 0x10, 0x20
+COMPAT_INLETRHO, COMPAT_EQ44, COMPAT_OMEGA4 = 0x10, 0x20, 0x40
 COMPAT_REFERENCE = (COMPAT_CREV20 | COMPAT_F32CONST | COMPAT_LIM210 | COMPAT_TRACE17 | COMPAT_INLETRHO |
-                    COMPAT_EQ44)
+                    COMPAT_EQ44 | COMPAT_OMEGA4)
 BC_WALL, BC_INLET_MOVING_WALL, BC_INLET_EQUILIBRIUM, BC_OUTLET_INTERPOLATED, BC_PERIODIC_GHOST = range(5)
 BC_OUTLET_EQUILIBRIUM, BC_OUTLET_NEIGHBOUR_EQ, BC_OUTLET_NEIGHBOUR_EQ_RF, BC_INLET_ZOU_HE = 5, 6, 7, 8
 

@@ -40,6 +40,7 @@ struct Consts {
     int lim210;   // the 210/012 limiters read |C_120 +- C_102| (reference quirk)
     int trace17;  // 2017 bulk term subtracts rho -+ 1 instead of k_000 (reference quirk)
     int eq44;     // 2017 Eq. 44 with D_x u + D_y v - 2 D_z w (reference quirk)
+    R w4q;        // omega_1^2 coefficient of omega_4's denominator (reference 8, published 9)
 };
 
 // Constants for a set of LBM_COMPAT_* bits.  Float literals of the reference
@@ -62,6 +63,7 @@ inline Consts<R> make_consts(uint32_t compat)
     K.lim210 = (compat & LBM_COMPAT_LIM210) != 0;
     K.trace17 = (compat & LBM_COMPAT_TRACE17) != 0;
     K.eq44 = (compat & LBM_COMPAT_EQ44) != 0;
+    K.w4q = (compat & LBM_COMPAT_OMEGA4) ? R(8) : R(9);
     return K;
 }
 
@@ -368,7 +370,7 @@ LBM_HD void cum_literal(const Consts<R>& K, R* f, R rho, R ux, R uy, R uz, R omg
         const R w210p012 = w3 + (R(1) - w3) * std::fabs(a210p) / (rho * lam + std::fabs(a210p));
         const R w201p021 = w3 + (R(1) - w3) * std::fabs(C201 + C021) / (rho * lam + std::fabs(C201 + C021));
         const R w4 = R(8) * (w1 - R(2)) * (w1 + w2 * (R(3) * w1 - R(7))) /
-                     (w2 * (R(56) - R(42) * w1 + R(8) * w1 * w1) - R(8) * w1);
+                     (w2 * (R(56) - R(42) * w1 + K.w4q * w1 * w1) - R(8) * w1);
         const R w120m102 = w4 + (R(1) - w4) * std::fabs(C120 - C102) / (rho * lam + std::fabs(C120 - C102));
         const R w210m012 = w4 + (R(1) - w4) * std::fabs(a210m) / (rho * lam + std::fabs(a210m));
         const R w201m021 = w4 + (R(1) - w4) * std::fabs(C201 - C021) / (rho * lam + std::fabs(C201 - C021));
@@ -674,7 +676,7 @@ LBM_HD void cum2017_fast(const Consts<R>& K, const R* fin, const Cen2<R>& c, R m
     // rates and limiters (Turbulent2017.h:240-291) with omega_2 = 1 substituted
     const R w1 = omg;
     const R w3 = -R(8) * (w1 - R(2)) * (R(2) * w1 + R(1)) * frcp(R(8) + w1 * (R(14) - R(7) * w1));
-    const R w4 = R(8) * (w1 - R(2)) * (R(4) * w1 - R(7)) * frcp(R(56) - R(50) * w1 + R(8) * w1 * w1);
+    const R w4 = R(8) * (w1 - R(2)) * (R(4) * w1 - R(7)) * frcp(R(56) - R(50) * w1 + K.w4q * w1 * w1);
     const R w5 = R(24) * (w1 - R(2)) * (R(2) + w1 * (R(7) - R(3) * w1)) *
                  frcp(R(16) * w1 * w1 * (w1 - R(6)) - R(2) * w1 * (R(216) + R(5) * w1 * (R(9) * w1 - R(46))) +
                       (w1 * (R(3) * w1 - R(10)) * (R(15) * w1 - R(28)) - R(48)));

@@ -100,7 +100,8 @@ int orc_get_threads(void)
 }
 
 /* Cumulant constants: 1.f/3.f, 2.f/3.f, 4.f/3.f and lambda = 0.01f
- * (CollisionCumD3Q27Turbulent2015.h:280-306, Turbulent2017.h:244-246, 349-362). */
+ * (CollisionCumD3Q27Turbulent2015.h:280-306, Turbulent2017.h:244-246, 349-362),
+ * and omega_4's 8 omega_1^2 (Turbulent2017.h:260; the published rate has 9). */
 #define ORC_FILL_CK(K, T)                                                                  \
     do {                                                                                   \
         const int f32c_ = (compat & LBM_COMPAT_F32CONST) != 0;                             \
@@ -108,6 +109,7 @@ int orc_get_threads(void)
         (K)->c23 = f32c_ ? (T)(2.f / 3.f) : (T)(2.0 / 3.0);                                \
         (K)->c43 = f32c_ ? (T)(4.f / 3.f) : (T)(4.0 / 3.0);                                \
         (K)->lam = f32c_ ? (T)0.01f : (T)0.01;                                             \
+        (K)->w4q = (compat & LBM_COMPAT_OMEGA4) ? (T)8 : (T)9;                             \
     } while (0)
 
 /* Weights: D3Q27.h:113-118 writes them as float quotients (8.f/27.f ...),
@@ -337,7 +339,7 @@ int orc_error_quadratic(orc* o, double* err)
 /* ---- single-cell entry points for unit tests ---- */
 
 #define ORC_CELL_CONSTS(T)                                                                  \
-    T w[27]; int crev[27]; T c13; struct { T c13, c23, c43, lam; } ck;                      \
+    T w[27]; int crev[27]; T c13; struct { T c13, c23, c43, lam, w4q; } ck;                     \
     for (int q = 0; q < 27; ++q) {                                                          \
         int k = orc_nnz(q);                                                                 \
         w[q] = (compat & LBM_COMPAT_F32CONST)                                               \
@@ -367,14 +369,14 @@ void orc_cell_collide(int collision, uint32_t compat, int use_double, const doub
     if (use_double) {
         ORC_CELL_CONSTS(double)
         (void)c13;
-        ck_d k = {ck.c13, ck.c23, ck.c43, ck.lam};
+        ck_d k = {ck.c13, ck.c23, ck.c43, ck.lam, ck.w4q};
         double o[27];
         collide_one_d(collision, compat, w, &k, flag, f, rho, ux, uy, uz, omega, o);
         for (int q = 0; q < 27; ++q) out[q] = o[q];
     } else {
         ORC_CELL_CONSTS(float)
         (void)c13;
-        ck_f k = {ck.c13, ck.c23, ck.c43, ck.lam};
+        ck_f k = {ck.c13, ck.c23, ck.c43, ck.lam, ck.w4q};
         float fi[27], o[27];
         for (int q = 0; q < 27; ++q) fi[q] = (float)f[q];
         collide_one_f(collision, compat, w, &k, flag, fi, (float)rho, (float)ux, (float)uy, (float)uz,

@@ -11,9 +11,11 @@
 
 /* Constants of the cumulant relaxation: 1/3, 2/3, 4/3 and the limiter
  * parameter lambda = 0.01 (float literals in the reference: 1.f/3.f, 2.f/3.f,
- * -4.f/3.f, 0.01f; exact values when the F32CONST quirk is off). */
+ * -4.f/3.f, 0.01f; exact values when the F32CONST quirk is off), and the
+ * omega_1^2 coefficient of omega_4's denominator (reference 8, published 9;
+ * LBM_COMPAT_OMEGA4). */
 typedef struct FN(ck) {
-    REAL c13, c23, c43, lam;
+    REAL c13, c23, c43, lam, w4q;
 } FN(ck);
 
 typedef struct FN(state) {
@@ -206,7 +208,7 @@ static void FN(cumulant)(int variant, int lim210, int trace17, int eq44, const F
         const REAL omega201p021 =
             omega3 + (1.f - omega3) * ORC_FABS(C_201 + C_021) / (rho * lambda3 + ORC_FABS(C_201 + C_021));
         const REAL omega4 = 8.f * (omega1 - 2.f) * (omega1 + omega2 * (3.f * omega1 - 7.f)) /
-                            (omega2 * (56.f - 42.f * omega1 + 8.f * omega1 * omega1) - 8.f * omega1);
+                            (omega2 * (56.f - 42.f * omega1 + K_->w4q * omega1 * omega1) - 8.f * omega1);
         const REAL omega120m102 =
             omega4 + (1.f - omega4) * ORC_FABS(C_120 - C_102) / (rho * lambda4 + ORC_FABS(C_120 - C_102));
         const REAL omega210m012 = omega4 + (1.f - omega4) * ORC_FABS(a210m) / (rho * lambda4 + ORC_FABS(a210m));

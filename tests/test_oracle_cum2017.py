@@ -18,10 +18,11 @@ brute-force definition; here the relaxation block is pinned:
   operator equals the 2015 one;
 - the Combined operator collides flag-1 cells with the 2015 operator and other
   fluid cells with the 2017 one;
-- physics: the optimised omega_3..5 and the B term of Geier 2017 make the
-  viscosity error of a shear wave less than half the 2015 operator's, whose
-  rates are all 1 (both second order).  The coefficients A and B are pinned
-  further in test_oracle_pins.py.
+- physics: with the published rates of Geier 2017 the shear-wave viscosity
+  error is fourth order; the reference's omega_4 (quirk LBM_COMPAT_OMEGA4)
+  makes it second order, still less than half the 2015 operator's, whose
+  rates are all 1.  The coefficients A and B are pinned further in
+  test_oracle_pins.py.
 """
 import numpy as np
 import pytest
@@ -219,22 +220,29 @@ def test_combined_lattice_with_interior_only_is_the_2015_operator():
     assert np.array_equal(out[0], out[1])
 
 
-def test_shear_wave_viscosity_error_is_smaller_than_2015():
+def test_shear_wave_viscosity_error_order():
     """Viscosity error of the axis-aligned shear wave u_y(x) (k || x), from a
-    fit of the log amplitude after the initial layer: both operators are
-    second order (error ratio 3.5-4.5 per halving of the wavelength), and the
-    2017 operator's optimised omega_3..5 and B make its error less than half
-    the 2015 operator's.  (A two-point estimate taken shortly after the
-    equilibrium start mixes in the initial layer and can suggest a faster
-    order; the fit does not.)"""
+    fit of the log amplitude after the initial layer.  The 2015 operator is
+    second order (error ratio 3.5-4.5 per halving of the wavelength).  The
+    2017 operator with the published omega_4 (Geier, Pasquali, Schoenherr
+    2017: omega_2 (56 - 42 omega_1 + 9 omega_1^2) - 8 omega_1 in the
+    denominator) is fourth order, as that paper derives it to be (ratio > 8);
+    with the reference's 8 omega_1^2 (LBM_COMPAT_OMEGA4) it falls back to
+    second order, its error still less than half the 2015 operator's.  (A
+    two-point estimate taken shortly after the equilibrium start mixes in the
+    initial layer and can suggest a faster order; the fit does not.)"""
     from test_oracle_pins import _wave_nu_error
     tau = 0.526   # omega_1 = 1.90: omega_3..5 = 0.41, 0.047, 0.16, all in (0, 2)
     e15 = [_wave_nu_error(L, tau, False, coll=C15) for L in (32, 64)]
     e17 = [_wave_nu_error(L, tau, False, coll=C17) for L in (32, 64)]
-    print(f"shear-wave nu error L=32,64: 2015 {e15[0]:.3e} {e15[1]:.3e}; 2017 {e17[0]:.3e} {e17[1]:.3e}")
-    for e in (e15, e17):
+    e17r = [_wave_nu_error(L, tau, False, coll=C17, compat=abi.COMPAT_OMEGA4) for L in (32, 64)]
+    print(f"shear-wave nu error L=32,64: 2015 {e15[0]:.3e} {e15[1]:.3e}; 2017 {e17[0]:.3e} {e17[1]:.3e}; "
+          f"2017 reference omega_4 {e17r[0]:.3e} {e17r[1]:.3e}")
+    for e in (e15, e17r):
         assert 3.5 < e[0] / e[1] < 4.5
-    assert abs(e17[0]) < 0.5 * abs(e15[0]) and abs(e17[1]) < 0.5 * abs(e15[1])
+    assert abs(e17[0] / e17[1]) > 8
+    assert abs(e17r[0]) < 0.5 * abs(e15[0]) and abs(e17r[1]) < 0.5 * abs(e15[1])
+    assert abs(e17[1]) < 0.05 * abs(e17r[1])
 
 
 def test_product_collision_arithmetic_is_bitwise_the_oracle_2017():
