@@ -288,6 +288,17 @@ static uint32_t step_seed(uint64_t t)
     return (uint32_t)(x >> 32);
 }
 
+// The boundary-aware step: 16-bit storage takes the kernel held to 7 waves
+// per SIMD, the other formats the plain entry point.
+template <class ST, int COLL, bool LIT, bool ODD, bool INL>
+static void launch_flagged(dim3 grid, dim3 block, hipStream_t s, const StepArgs<ST>& a)
+{
+    if constexpr (sizeof(typename ST::T) == 2)
+        k_step_w7<ST, COLL, LIT, ODD, INL><<<grid, block, 0, s>>>(a);
+    else
+        k_step<ST, COLL, LIT, ODD, true, false, INL><<<grid, block, 0, s>>>(a);
+}
+
 template <class ST, int COLL, bool LIT>
 static lbm_status launch_steps(lbm_ctx* c, int nsteps)
 {
@@ -369,11 +380,11 @@ static lbm_status launch_steps(lbm_ctx* c, int nsteps)
         }
         if (!mlp) {
             if (c->flags_needed && c->inl_deferred) {
-                if (odd) k_step<ST, COLL, LIT, true, true, false, true><<<grid, block, 0, c->stream>>>(a);
-                else k_step<ST, COLL, LIT, false, true, false, true><<<grid, block, 0, c->stream>>>(a);
+                if (odd) launch_flagged<ST, COLL, LIT, true, true>(grid, block, c->stream, a);
+                else launch_flagged<ST, COLL, LIT, false, true>(grid, block, c->stream, a);
             } else if (c->flags_needed) {
-                if (odd) k_step<ST, COLL, LIT, true, true><<<grid, block, 0, c->stream>>>(a);
-                else k_step<ST, COLL, LIT, false, true><<<grid, block, 0, c->stream>>>(a);
+                if (odd) launch_flagged<ST, COLL, LIT, true, false>(grid, block, c->stream, a);
+                else launch_flagged<ST, COLL, LIT, false, false>(grid, block, c->stream, a);
             } else {
                 if (odd) k_step<ST, COLL, LIT, true, false><<<grid, block, 0, c->stream>>>(a);
                 else k_step<ST, COLL, LIT, false, false><<<grid, block, 0, c->stream>>>(a);

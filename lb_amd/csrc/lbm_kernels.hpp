@@ -528,8 +528,8 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
 // INL: inlet rules deferred into this kernel (a separate instantiation: the
 // rule's code, even untaken, cost the obstacle workload's flagged step 1-2 %
 // in FP32/FP64 and 8 % in FP16 storage).
-template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false, bool INL = false>
-__global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
+template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC, bool INL>
+__device__ __forceinline__ void step_cells(const StepArgs<ST>& a)
 {
     using R = typename ST::R;
     const Cell c = cell_of(a.g);
@@ -786,6 +786,23 @@ __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
             a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
         }
     }
+}
+
+template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false, bool INL = false>
+__global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
+{
+    step_cells<ST, COLL, LIT, ODD, FLAGS, CC, INL>(a);
+}
+
+// The boundary-aware step for 16-bit storage, held to 7 waves per SIMD (72
+// VGPRs instead of 75-76, which allow 6).  A 16-bit wave moves half the bytes
+// of an FP32 one, so it needs more waves in flight: on the obstacle workload
+// the step took 2 % less time.  FP32 was unchanged, and 8 waves spilled and
+// ran 7 % slower.
+template <class ST, int COLL, bool LIT, bool ODD, bool INL = false>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(7, 7))) void k_step_w7(StepArgs<ST> a)
+{
+    step_cells<ST, COLL, LIT, ODD, true, false, INL>(a);
 }
 
 // Moments, Smagorinsky rate and closure features of one cell, shared by the
