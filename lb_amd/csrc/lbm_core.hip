@@ -48,7 +48,7 @@ struct lbm_ctx {
     NodeDev* outl = nullptr;
     NodeDev* ghost = nullptr;
     void* pub_mac = nullptr;
-    void* inl_post = nullptr;     // hand-over to k_inlet_state: f* and rho, u
+    void* inl_post = nullptr;     // hand-over to the inlet rule (k_inlet_state, or deferred): f* and rho, u
     void* inl_mac = nullptr;
     void* pub_post = nullptr;
     void* macro = nullptr;     // 4 planes, arithmetic type, padded layout

@@ -690,7 +690,8 @@ __device__ __forceinline__ void step_cells(const StepArgs<ST>& a)
         // post-collision values die as they are stored: bounce-back and kept
         // links, the data the patch kernels of this step read, and for
         // moving-wall and Zou-He inlets the collision their state update
-        // (k_inlet_state, after this kernel) starts from.  None of these
+        // (k_inlet_state after this kernel, or on a deferred lid the next
+        // step's kernel) starts from.  None of these
         // writes touches a slot the streaming writes use.  Inlet cells then
         // take their previous state into P: that is what they stream.
         if (FLAGS && cls != CL_FLUID && inlet) {
