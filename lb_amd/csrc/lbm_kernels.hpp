@@ -531,268 +531,24 @@ __device__ __forceinline__ void stream_cell(const StepArgs<ST>& a, const Cell& c
 template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC, bool INL>
 __device__ __forceinline__ void step_cells(const StepArgs<ST>& a)
 {
-    using R = typename ST::R;
-    const Cell c = cell_of(a.g);
-    bool active = c.inside;
-    uint8_t cls = CL_FLUID;
-    R f[Q];
-    if (FLAGS && active) {
-        // the populations are loaded with the class byte, not after it: every
-        // cell of the caller's lattice has valid read slots, and the few
-        // solid and inlet cells drop what they loaded (0.4-0.9 % faster on
-        // the obstacle workload than loading after the class is known)
-        cls = a.cls[c.i];
-        load_pop<ST, ODD>(a.p, a.rest, c, f);
-    }
-    if (cls == CL_SOLID) active = false;
-    double dm[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // mass, momentum, wall force
-
-    if (active) {
-        int bi = -1;
-        const bool inlet = FLAGS && cls == CL_INLET;
-        if (FLAGS && cls >= CL_INLET) bi = a.bcidx[c.i];
-        if (inlet && !INL) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = a.inl_state[(size_t)bi * Q + q];
-        } else if (inlet) {
-            R* stp = a.inl_state + (size_t)bi * Q;
-#pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = stp[q];
-            if (a.inl_pending) {
-                // deferred inlet: the moving-wall or Zou-He rule of the
-                // previous step, from that step's records, applied to the
-                // stored state and stored (this cell streams it below, and
-                // read-backs see it)
-                inlet_rule(a.K, a.inl[bi], a.inl_post + (size_t)bi * Q, a.inl_mac + (size_t)bi * PUB_MAC,
-                           a.crev20 != 0, f);
-#pragma unroll
-                for (int q = 0; q < Q; ++q) stp[q] = f[q];
-            }
-        } else {
-            if (!FLAGS) load_pop<ST, ODD>(a.p, a.rest, c, f);
-            // deferred interpolated outlet (4- and 8-byte storage only): the
-            // rule of the previous step applied to the state just loaded
-            if constexpr (sizeof(typename ST::T) >= 4) {
-                if (FLAGS && cls == CL_OUTL && a.outl_pending) {
-                    interp_outlet(f, a.outl[bi], a.pub_prev, a.K, a.cs, a.crev20 != 0);
-                    // incoming populations without a streaming source keep
-                    // their value, which the boundary path reads again from
-                    // the read slot after the collision: the rule's result
-                    // goes there, as the patch kernel would have stored it
-                    // (only this cell reads or writes those slots this step)
-                    const uint64_t nwo = a.node[c.i];
-                    if ((nwo >> 63) == 0) {
-                        const uint32_t So = (uint32_t)(nwo & MASK27);
-#pragma unroll
-                        for (int q = 0; q < Q; ++q)
-                            if ((So >> q) & 1u)
-                                ST::store(a.p[read_plane(ODD, q)], read_index(c, ODD, q), f[q], a.rest[WCLASS[q]]);
-                    }
-                }
-            }
-        }
-
-        // the Combined operator reads the node word before colliding: bit 62
-        // marks the cells that take the 2017 branch
-        const uint64_t nw17 =
-            (COLL == LBM_COLLISION_COMBINED && FLAGS && cls != CL_FLUID) ? a.node[c.i] : 0ull;
-        const bool c17 = COLL == LBM_COLLISION_CUM2017 || ((nw17 >> NW_CUM2017) & 1ull);
-
-        R rho, ux, uy, uz, omega;
-        R P[Q];
-        if (LIT) {
-            if (a.first) {
-                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-                rho = a.macro[c.i];
-                ux = a.macro[N + c.i];
-                uy = a.macro[2 * N + c.i];
-                uz = a.macro[3 * N + c.i];
-            } else {
-                moments_literal(f, rho, ux, uy, uz);
-            }
-            if (a.avg)
-                moment_add(a.avg, (size_t)a.g.ux * a.g.uy * a.g.uz, user_index(a.g, c), (double)rho, (double)ux,
-                           (double)uy, (double)uz);
-            omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_literal(a.K, f, rho, ux, uy, uz, a.tau, a.cles)
-                                                         : a.omega0;
-            omega = sponge(omega, a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) P[q] = f[q];
-            if (COLL == LBM_COLLISION_SRT)
-                srt_literal(a.K, P, rho, ux, uy, uz, omega);
-            else if (!c17)
-                cum_literal<CV_2015>(a.K, P, rho, ux, uy, uz, omega);
-            else if (COLL == LBM_COLLISION_CUM2017)
-                cum_literal<CV_2017>(a.K, P, rho, ux, uy, uz, omega);
-            else
-                cum_literal<CV_2017C>(a.K, P, rho, ux, uy, uz, omega);
-            if (a.diag) {
-                dm[0] = (double)rho;
-                dm[1] = (double)(rho * ux);
-                dm[2] = (double)(rho * uy);
-                dm[3] = (double)(rho * uz);
-            }
-        } else {
-            const Raw2<R> r = raw_moments(f);
-            if (a.first) {
-                const size_t N = (size_t)a.g.nx * a.g.ny * a.g.nz;
-                rho = a.macro[c.i];
-                ux = a.macro[N + c.i];
-                uy = a.macro[2 * N + c.i];
-                uz = a.macro[3 * N + c.i];
-            } else {
-                rho = r.m0;
-                const R ir = fast_rcp(rho);
-                ux = r.jx * ir;
-                uy = r.jy * ir;
-                uz = r.jz * ir;
-            }
-            if (a.avg) {
-                // the values the macroscopic read-back forms (u = j / rho)
-                const bool fs = a.first != 0;
-                moment_add(a.avg, (size_t)a.g.ux * a.g.uy * a.g.uz, user_index(a.g, c), (double)rho,
-                           (double)(fs ? ux : r.jx / rho), (double)(fs ? uy : r.jy / rho),
-                           (double)(fs ? uz : r.jz / rho));
-            }
-            const Cen2<R> ce = central2(r, ux, uy, uz);
-            if (CC)   // learned closure between refreshes: held coefficient, current stress
-                omega = omega_from_coefficient((R)a.ccache[c.i], pin_over_rho(ce, rho), a.tau);
-            else
-                omega = a.closure == LBM_CLOSURE_SMAGORINSKY ? omega_les_fast(ce, rho, a.tau, a.cles) : a.omega0;
-            omega = sponge(omega, a.sp_axis, a.sp_begin, a.sp_deriv, a.g, c.x, c.y, c.z);
-            if (COLL == LBM_COLLISION_SRT) {
-#pragma unroll
-                for (int q = 0; q < Q; ++q) P[q] = f[q];
-                srt_fast(a.K, P, rho, ux, uy, uz, omega);
-            } else if (!c17) {
-                cum2015_fast(a.K, ce, r.m0, rho, ux, uy, uz, omega, P);
-            } else if (COLL == LBM_COLLISION_CUM2017) {
-                cum2017_fast<CV_2017>(a.K, f, ce, r.m0, rho, ux, uy, uz, omega, P);
-            } else {
-                cum2017_fast<CV_2017C>(a.K, f, ce, r.m0, rho, ux, uy, uz, omega, P);
-            }
-            if (a.diag) {
-                dm[0] = (double)r.m0;
-                dm[1] = (double)r.jx;
-                dm[2] = (double)r.jy;
-                dm[3] = (double)r.jz;
-            }
-        }
-
-        const uint32_t hsr = ST::sr ? sr_hash(c.i, a.seed) : 0u;
-        // node word of a boundary cell (0 for plain fluid): skip masks and walls
-        const uint64_t nw =
-            (FLAGS && cls != CL_FLUID) ? (COLL == LBM_COLLISION_COMBINED ? nw17 : a.node[c.i]) : 0ull;
-        const uint32_t S = (uint32_t)(nw & MASK27);
-        const uint32_t Kk = (uint32_t)((nw >> 32) & MASK27);
-        const bool wall = (nw >> 63) != 0;
-        // Boundary work comes before the streaming writes, so that the
-        // post-collision values die as they are stored: bounce-back and kept
-        // links, the data the patch kernels of this step read, and for
-        // moving-wall and Zou-He inlets the collision their state update
-        // (k_inlet_state after this kernel, or on a deferred lid the next
-        // step's kernel) starts from.  None of these
-        // writes touches a slot the streaming writes use.  Inlet cells then
-        // take their previous state into P: that is what they stream.
-        if (FLAGS && cls != CL_FLUID && inlet) {
-            const int kind = a.inl[bi].kind;
-            if (kind == LBM_BC_INLET_MOVING_WALL || kind == LBM_BC_INLET_ZOU_HE) {
-                R* m = a.inl_mac + (size_t)bi * PUB_MAC;
-                m[0] = rho;
-                m[1] = ux;
-                m[2] = uy;
-                m[3] = uz;
-                R* dst = a.inl_post + (size_t)bi * Q;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) dst[q] = P[q];
-            }
-            // an inlet cell streams its pre-collision state (StreamingAB.h:62-65)
-#pragma unroll
-            for (int q = 0; q < Q; ++q) P[q] = a.inl_state[(size_t)bi * Q + q];
-        }
-        if (FLAGS && cls != CL_FLUID && !inlet) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                if ((S >> q) & 1u) {
-                    R v;
-                    if (!wall)
-                        v = reload_pop<ST, ODD>(a.p, a.rest, c, q);
-                    else if (q == 20)
-                        v = a.crev20 ? P[21] : P[19];
-                    else
-                        v = P[OPP[q]];
-                    store_in_next<ST, ODD>(a.p, a.rest, c, q, v, hsr);
-                    if (wall && a.diag) {
-                        // momentum the fluid loses across this link: the
-                        // outgoing f*_{opp q} leaves, the bounced v enters
-                        const double m = (double)v + (double)P[OPP[q]];
-                        dm[4] -= CX[q] * m;
-                        dm[5] -= CY[q] * m;
-                        dm[6] -= CZ[q] * m;
-                    }
-                }
-            }
-            // data the boundary patch kernels of this step (and, for deferred
-            // outlet cells, the next step) read
-            if (cls == CL_PUB || cls == CL_OUTL) {
-                const int ps = cls == CL_PUB ? bi : a.outl[bi].s0;
-                a.pub_mac[(size_t)ps * PUB_MAC] = rho;
-                a.pub_mac[(size_t)ps * PUB_MAC + 1] = ux;
-                a.pub_mac[(size_t)ps * PUB_MAC + 2] = uy;
-                a.pub_mac[(size_t)ps * PUB_MAC + 3] = uz;
-                R* dst = a.pub_post + (size_t)ps * Q;
-#pragma unroll
-                for (int q = 0; q < Q; ++q) dst[q] = P[q];
-            }
-        }
-        // streaming writes, shared by every lane of the wave (boundary lanes drop
-        // the writes their mask skips).  On the odd step their 27 neighbour
-        // offsets are those of the loads; formed once and kept through the
-        // collision they held 13-20 more registers (one wave per SIMD less),
-        // so they are formed again from an opaque copy of the cell.
-        Cell cst = c;
-        if constexpr (ODD)
-            __asm__ volatile("" : "+v"(cst.i), "+v"(cst.dxm), "+v"(cst.dxp), "+v"(cst.dym), "+v"(cst.dyp),
-                             "+v"(cst.dzm), "+v"(cst.dzp));
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, true>(a.p, a.rest, cst, q, P[q], hsr);
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (!line_aligned(ODD, q) && !((Kk >> q) & 1u))
-                store_out<ST, ODD, false>(a.p, a.rest, cst, q, P[q], hsr);
-    }
-
-    if (a.diag) {
-        // wave-level FP64 reduction, then across the 4 waves of the block
-        __shared__ double red[TPB / 64][7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-            double v = dm[k];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            dm[k] = v;
-        }
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 7; ++k) red[wv][k] = dm[k];
-        }
-        __syncthreads();
-        if (threadIdx.x < 7) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
-            a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = s;
-        }
-    }
+#include "step_body.inc"
 }
 
+// The step kernel.  For FP32 storage it runs the body through step_cells;
+// for the 2- and 8-byte formats the body is compiled in the kernel itself.
+// The two forms are the same source but compile to different schedules: the
+// even FP32 step through step_cells computes all 27 post-collision values
+// before its stores (107 VGPRs) and ran 2.9 % faster on the periodic 256^3
+// workload (cumulant 2017: 3.2 %), while the 16-bit steps ran 2.1-2.6 %
+// slower that way and FP64 0.5 %.
 template <class ST, int COLL, bool LIT, bool ODD, bool FLAGS, bool CC = false, bool INL = false>
 __global__ __launch_bounds__(TPB) void k_step(StepArgs<ST> a)
 {
-    step_cells<ST, COLL, LIT, ODD, FLAGS, CC, INL>(a);
+    if constexpr (sizeof(typename ST::T) == 4) {
+        step_cells<ST, COLL, LIT, ODD, FLAGS, CC, INL>(a);
+    } else {
+#include "step_body.inc"
+    }
 }
 
 // The boundary-aware step for 16-bit storage, held to 7 waves per SIMD (72
